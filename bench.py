@@ -1,0 +1,236 @@
+#!/usr/bin/env python3
+"""MDAS sampler benchmark on one MI355X.
+
+A step is one full MDAS run over one synthetic 1080p image: initial Morton
+grid, adaptive passes (error, selection, best-candidate placement, median
+split) with the analytic integrand between them, until N >= spp * W * H
+(optixMotionBlur.cpp:768-791), then the reconstruction to pixels.
+
+value = final samples / GPU time of the MDAS phases (initial + errors +
+adaptive + reconstruction), the reference's "total sampling time"
+(SDK/data/Scripts/breakdown_table.py:58-66); the integrand (the OptiX trace
+in the reference) is timed separately and reported as e2e_msamples_per_s.
+
+Prints one JSON line.  Usage:
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--workload NAME]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOADS = {
+    # 5D motion blur + depth of field, 1920x1080, 32 spp (64 moving spheres)
+    "mdas-5d-mbdof-1080p-32spp": dict(dim=5, kind=3, W=1920, H=1080, spp=32, b=1, e=8,
+                                      alpha=0.25, cand=4, seed=1),
+    # 4D depth of field, 1920x1080, 16 spp (64 static spheres)
+    "mdas-4d-dof-1080p-16spp": dict(dim=4, kind=2, W=1920, H=1080, spp=16, b=1, e=8,
+                                    alpha=0.25, cand=4, seed=1),
+}
+DEFAULT = "mdas-5d-mbdof-1080p-32spp"
+HBM_PEAK_GBS = 8000.0
+
+
+def capacity(w):
+    target = int(w["spp"] * w["W"] * w["H"])
+    l0 = 1 << (w["b"] * w["dim"] + 2 * w["e"])
+    return target, int(target * 1.5) + 4 * l0 + 1024
+
+
+def run_gpu(name, steps, warmup, log=print):
+    from pmdas_amd import hip, kdgrid
+    w = WORKLOADS[name]
+    target, cap = capacity(w)
+    g = kdgrid.KDGrid(w["dim"], cap, w["cand"], w["b"], w["e"], w["alpha"], float(w["W"]),
+                      float(w["H"]))
+    sc = kdgrid.Scene(w["kind"], w["seed"], w["dim"], float(w["W"]), float(w["H"]))
+    px = hip.DeviceBuffer(w["W"] * w["H"] * 16)
+    g.set_timing(True)
+    for i in range(warmup):
+        g.Run(sc, target)
+        g.Integrate(px.ptr, w["W"], w["H"])
+        hip.synchronize()
+        log(f"[bench] {name}: warmup {i + 1}/{warmup} done")
+    tot_mdas = tot_wall = tot_integrand = 0.0
+    samples = 0
+    kstats = {}
+    phases = None
+    for i in range(steps):
+        hip.synchronize()
+        t0 = time.perf_counter()
+        g.Run(sc, target)
+        g.Integrate(px.ptr, w["W"], w["H"])
+        hip.synchronize()
+        t1 = time.perf_counter()
+        ph = g.phase_times()
+        mdas_ms = ph["initial_ms"] + ph["errors_ms"] + ph["adaptive_ms"] + ph["recon_ms"]
+        tot_mdas += mdas_ms
+        tot_integrand += ph["integrand_ms"]
+        tot_wall += (t1 - t0) * 1e3
+        samples += g.GetNumberOfSamples()
+        for k, (la, ms, by) in g.kernel_stats().items():
+            a = kstats.setdefault(k, [0, 0.0, 0.0])
+            a[0] += la
+            a[1] += ms
+            a[2] += by
+        phases = ph
+        log(f"[bench] {name}: step {i + 1}/{steps}: {mdas_ms:.2f} ms MDAS, "
+            f"{ph['integrand_ms']:.2f} ms integrand, {(t1 - t0) * 1e3:.2f} ms wall, "
+            f"{ph['passes']} passes, N={g.GetNumberOfSamples()}")
+    res = dict(
+        workload=name, target=target, samples=samples // steps,
+        leaves=g.GetNumberOfNodes(), passes=phases["passes"],
+        mdas_ms=tot_mdas / steps, integrand_ms=tot_integrand / steps, wall_ms=tot_wall / steps,
+        msamples_per_s=samples / (tot_mdas * 1e3),
+        e2e_msamples_per_s=samples / (tot_wall * 1e3),
+        phases_ms={k: phases[k] for k in ("initial_ms", "errors_ms", "adaptive_ms", "recon_ms",
+                                          "sort_ms", "integrand_ms")},
+        kernels={k: dict(launches=v[0] // steps, ms=v[1] / steps, bytes=v[2] / steps)
+                 for k, v in kstats.items()},
+    )
+    px.free()
+    g.close()
+    sc.close()
+    return res
+
+
+def roofline(res, traffic_file):
+    """Dominant MDAS kernel (integrand excluded): algorithmic bytes / its time."""
+    ks = {k: v for k, v in res["kernels"].items() if k != "k_eval"}
+    name = max(ks, key=lambda k: ks[k]["ms"])
+    k = ks[name]
+    achieved = k["bytes"] / (k["ms"] * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            t = json.load(f)
+        if t.get("kernel") == name:
+            traffic = t.get("hbm_bytes_per_launch")
+    return dict(bound="hbm", kernel=name, achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
+                unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                algorithmic_bytes_per_launch=k["bytes"] / max(k["launches"], 1),
+                avg_launch_ms=k["ms"] / max(k["launches"], 1), launches=k["launches"])
+
+
+def cpu_baseline(name, shrink=0, log=print):
+    """The OpenMP oracle on the host cores, MDAS phases only (integrand
+    excluded, as on the GPU).  shrink = k runs a (W/2^k) x (H/2^k) version of
+    the workload with extraImgBits - k, which keeps 4 initial samples per
+    initial cell and the same spp."""
+    from oracle import pyoracle
+    w = dict(WORKLOADS[name])
+    w["W"] >>= shrink
+    w["H"] >>= shrink
+    w["e"] -= shrink
+    target, cap = capacity(w)
+    lib = pyoracle.load()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    threads = min(threads, 16)
+    lib.orc_set_threads(threads)
+    o = pyoracle.OracleGrid(w["dim"], cap, w["cand"], w["b"], w["e"], w["alpha"], float(w["W"]),
+                            float(w["H"]))
+    params = pyoracle.make_scene(w["kind"], w["seed"], w["dim"], float(w["W"]), float(w["H"]))
+    t_mdas = 0.0
+    t0 = time.perf_counter()
+    o.initial_sampling()
+    t_mdas += time.perf_counter() - t0
+    o.eval_scene(params, 0, o.n_samples)
+    passes = 0
+    while o.n_samples < target:
+        t0 = time.perf_counter()
+        o.compute_errors()
+        rc = o.adaptive_sampling()
+        t_mdas += time.perf_counter() - t0
+        if rc:
+            raise RuntimeError("oracle capacity")
+        o.eval_scene(params, o.n_samples - o.new_samples, o.new_samples)
+        passes += 1
+    t0 = time.perf_counter()
+    o.integrate(w["W"], w["H"])
+    t_mdas += time.perf_counter() - t0
+    n = o.n_samples
+    o.close()
+    log(f"[bench] cpu baseline: {n} samples in {t_mdas:.2f} s MDAS time, {passes} passes, "
+        f"{threads} threads")
+    return dict(value=round(n / t_mdas / 1e6, 3), unit="Msamples/s", cores=threads, kind="port",
+                sample=(f"full workload {name}" if shrink == 0 else
+                        f"{w['W']}x{w['H']} version of {name} (extraImgBits {w['e']})")
+                + f", {n} samples, {passes} passes, MDAS phases only, one run")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--cpu-shrink", type=int, default=0,
+                    help="run the CPU baseline on a 2^-k per-axis version of the workload")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (the sampler is single-GPU; "
+              "sharding by image / screen tile is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    import pmdas_amd
+    from pmdas_amd import hip
+    pmdas_amd.load()
+    if hip.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    res = run_gpu(a.workload, a.steps, a.warmup, log)
+    traffic_file = os.path.join(ROOT, "profiles", f"traffic_{a.workload}.json")
+    line = {
+        "metric": "Msamples/s (build+error+sample+recon); time-to-equal-MSE learned vs plain",
+        "value": round(res["msamples_per_s"], 2),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(res["wall_ms"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded analytic scene)",
+        "config": {"workload": a.workload, "width": WORKLOADS[a.workload]["W"],
+                   "height": WORKLOADS[a.workload]["H"], "dim": WORKLOADS[a.workload]["dim"],
+                   "spp": WORKLOADS[a.workload]["spp"],
+                   "bits_per_dim": WORKLOADS[a.workload]["b"],
+                   "extra_img_bits": WORKLOADS[a.workload]["e"],
+                   "alpha": WORKLOADS[a.workload]["alpha"],
+                   "candidates_num": WORKLOADS[a.workload]["cand"],
+                   "parallelism": "single GPU"},
+        "mdas_ms_per_step": round(res["mdas_ms"], 3),
+        "integrand_ms_per_step": round(res["integrand_ms"], 3),
+        "e2e_msamples_per_s": round(res["e2e_msamples_per_s"], 2),
+        "samples": res["samples"], "leaves": res["leaves"], "passes": res["passes"],
+        "phases_ms": {k: round(v, 3) for k, v in res["phases_ms"].items()},
+        "roofline": roofline(res, traffic_file),
+        "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 4),
+                        "GBps": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)}
+                    for k, v in sorted(res["kernels"].items(), key=lambda kv: -kv[1]["ms"])},
+        "device": hip.device_name(),
+    }
+    if not a.no_secondary:
+        other = [k for k in WORKLOADS if k != a.workload][0]
+        r2 = run_gpu(other, max(1, min(3, a.steps)), 1, log)
+        line["secondary"] = {"workload": other, "value": round(r2["msamples_per_s"], 2),
+                             "unit": "Msamples/s", "mdas_ms_per_step": round(r2["mdas_ms"], 3),
+                             "e2e_msamples_per_s": round(r2["e2e_msamples_per_s"], 2),
+                             "samples": r2["samples"], "passes": r2["passes"]}
+    if not a.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_shrink, log)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,175 @@
+/*
+ * mdas.h -- C ABI of the MI355X-native MDAS sampler (libmdas.so).
+ *
+ * Drop-in boundary for the reference's C++ class mdas::KDGrid<Point>
+ * (SDK/lib/mdas/kdgrid.h:10-107, explicitly instantiated for Point3..Point6 at
+ * SDK/lib/mdas/kdgrid.cu:25-104) as the sample apps call it
+ * (SDK/optixMotionBlur/optixMotionBlur.cpp:331-360, 473-491, 768-806).
+ * Plain pointers and sizes only; a HIP stream is passed as `void*`
+ * (NULL = the null stream, as the reference uses stream 0).
+ *
+ * Conventions (differences from the reference are deliberate and listed):
+ *  - every entry point returns an int status (MDAS_OK = 0); the message of the
+ *    last failure on the calling thread is mdas_last_error().  The reference
+ *    returns void and only prints CUDA errors (buffer.h:6-16).
+ *  - all state is per handle (the reference keeps g_error, g_warpCounter0/1 and
+ *    a texture reference in device globals, kdgrid.cu:19-23).
+ *  - buffers are device memory (hipMalloc); the reference uses managed memory
+ *    (buffer.h:44).  Coordinates are AoS float[dim] per sample and values are
+ *    float3 per sample, exactly as the reference's LaunchParams expect
+ *    (SDK/cuda/whitted.h:76-81).
+ *  - a pass that would exceed max_samples returns MDAS_ERR_CAPACITY and leaves
+ *    samples and tree unchanged (the reference writes out of bounds).
+ *  - indices are allocated in leaf order by a device-wide scan, so results are
+ *    deterministic (the reference's warp atomics are one of many orders).
+ */
+#ifndef MDAS_H
+#define MDAS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDAS_OK 0
+#define MDAS_ERR_INVALID (-1)   /* bad argument or configuration */
+#define MDAS_ERR_CAPACITY (-2)  /* a pass would exceed max_samples */
+#define MDAS_ERR_HIP (-3)       /* HIP runtime error */
+#define MDAS_ERR_STATE (-4)     /* call out of order (e.g. no initial sampling) */
+
+typedef struct mdas_grid* mdas_handle;
+typedef struct mdas_scene* mdas_scene_handle;
+
+/* Constructor arguments of KDGrid (kdgrid.h:42-51; kdgrid.cu:600-631) and the
+ * app's Mdas.* options (optixMotionBlur.cpp:118-125, 336-350). */
+typedef struct mdas_config {
+    int dim;                /* Point::DIM, 3..6 */
+    long long max_samples;  /* sample capacity and stopping cap (<= 2^31-1) */
+    int candidates_num;     /* Mdas.candidatesNum (default 4) */
+    int bits_per_dim;       /* Mdas.bitsPerDim */
+    int extra_img_bits;     /* Mdas.extraImgBits */
+    float alpha;            /* Mdas.alpha */
+    float scale_x;          /* Mdas.scaleFactor * Film.width */
+    float scale_y;          /* Mdas.scaleFactor * Film.height */
+    const char* log_path;   /* optional "KEY\nvalue" log (kdgrid.cu:670-817); NULL = off */
+} mdas_config;
+
+/* KDGrid::KDGrid (kdgrid.h:42-51) */
+int mdas_create(const mdas_config* cfg, mdas_handle* out);
+/* KDGrid::~KDGrid (buffers freed; buffer.h:27-29) */
+int mdas_destroy(mdas_handle h);
+
+/* KDGrid::InitialSampling (kdgrid.h:53; kdgrid.cu:633-674) */
+int mdas_initial_sampling(mdas_handle h, void* stream);
+/* KDGrid::ComputeErrors (kdgrid.h:55; kdgrid.cu:676-716) */
+int mdas_compute_errors(mdas_handle h, void* stream);
+/* KDGrid::AdaptiveSampling (kdgrid.h:56; kdgrid.cu:718-773).  Like the
+ * reference it synchronises to read the new sample / node counts. */
+int mdas_adaptive_sampling(mdas_handle h, void* stream);
+/* KDGrid::SamplingPass (kdgrid.h:58; kdgrid.cu:775-779) */
+int mdas_sampling_pass(mdas_handle h, void* stream);
+/* KDGrid::Integrate (kdgrid.h:59; kdgrid.cu:781-820).  `pixels` is a device
+ * float4[width*height]; every pixel is written (xyz = estimate, w = 0). */
+int mdas_integrate(mdas_handle h, float* pixels, int width, int height, void* stream);
+/* Reference-style reconstruction with float atomics (kdgrid.cu:483-582);
+ * same result up to summation order.  Kept for comparison. */
+int mdas_integrate_scatter(mdas_handle h, float* pixels, int width, int height, void* stream);
+/* KDGrid::ConvertToBytes (kdgrid.h:61; kdgrid.cu:839-846) -> uchar4 sRGB */
+int mdas_convert_to_bytes(mdas_handle h, const float* pixels, unsigned char* bytes, int width,
+                          int height, void* stream);
+/* KDGrid::SamplingDensity (kdgrid.h:60; kdgrid.cu:822-837): host float4 image */
+int mdas_sampling_density(mdas_handle h, float* host_pixels, int width, int height);
+/* KDGrid::Validate (kdgrid.h:63; kdgrid.cu:848-961), always enabled; *valid = 0/1 */
+int mdas_validate(mdas_handle h, int* valid);
+
+/* Getters (kdgrid.h:65-77).  Pointers are device pointers. */
+float* mdas_get_sample_coordinates(mdas_handle h);
+float* mdas_get_sample_values(mdas_handle h);
+int mdas_get_dim(mdas_handle h);
+int mdas_get_candidates_num(mdas_handle h);
+int mdas_get_max_node_size(mdas_handle h);
+int mdas_get_new_samples(mdas_handle h);
+int mdas_get_new_nodes(mdas_handle h);
+int mdas_get_number_of_samples(mdas_handle h);
+int mdas_get_number_of_nodes(mdas_handle h);
+int mdas_get_max_samples(mdas_handle h);
+float mdas_get_scale_x(mdas_handle h);
+float mdas_get_scale_y(mdas_handle h);
+float mdas_get_max_error(mdas_handle h);
+
+/* ---- additions without a reference counterpart ---------------------------
+ * Analytic integrands replacing the OptiX raygen programs
+ * (SDK/cuda/whitted.cu:164-293, ao.cu:112-152) over the same buffer contract. */
+#define MDAS_SCENE_CONSTANT 0
+#define MDAS_SCENE_DISKS_MB 1
+#define MDAS_SCENE_SPHERES_DOF 2
+#define MDAS_SCENE_SPHERES_MBDOF 3
+#define MDAS_SCENE_AO 4
+#define MDAS_SCENE_RAMP 5
+int mdas_scene_create(int kind, uint32_t seed, int dim, float scale_x, float scale_y,
+                      mdas_scene_handle* out);
+int mdas_scene_destroy(mdas_scene_handle s);
+/* host copy of the scene parameter block; returns its size in bytes */
+long long mdas_scene_params(mdas_scene_handle s, void* out, long long capacity);
+/* Integrand over samples [offset, offset+count) -- what optixLaunch(width =
+ * sample_count) does with sample_offset (optixMotionBlur.cpp:458-467). */
+int mdas_eval_integrand(mdas_handle h, mdas_scene_handle s, int offset, int count, void* stream);
+/* Integrand over the slice of the last adaptive pass, read on the device. */
+int mdas_eval_new_samples(mdas_handle h, mdas_scene_handle s, void* stream);
+
+/* Whole sampling loop without per-pass host round trips: initial sampling,
+ * integrand, then passes while N < min(target_samples, max_samples)
+ * (optixMotionBlur.cpp:768-791).  The stopping rule is evaluated on the
+ * device; the host checks it once every `passes_per_sync` passes. */
+int mdas_run(mdas_handle h, mdas_scene_handle s, long long target_samples, void* stream);
+int mdas_get_passes(mdas_handle h);
+
+/* Per-phase GPU time of the last mdas_run / mdas_integrate (ms), from HIP
+ * events on the caller's stream: initial, errors, adaptive, integrand,
+ * reconstruction (incl. sort), bytes.  Enabled by mdas_set_timing(h, 1). */
+typedef struct mdas_phase_times {
+    double initial_ms, errors_ms, adaptive_ms, integrand_ms, recon_ms, sort_ms;
+    int passes;
+    long long final_samples, final_leaves;
+    double bytes_initial, bytes_errors, bytes_adaptive, bytes_recon;  /* algorithmic */
+} mdas_phase_times;
+int mdas_set_timing(mdas_handle h, int enable);
+int mdas_get_phase_times(mdas_handle h, mdas_phase_times* out);
+
+/* Per-pass device counters of the last run, 8 ints per row: leaves and
+ * samples before the pass, selected, split, samples held by the selected
+ * leaves, stale leaves re-estimated, values they read, 0.  Returns the rows. */
+int mdas_get_pass_stats(mdas_handle h, int* out, int max_rows);
+
+/* Per-kernel GPU time of the last mdas_run + mdas_integrate when timing is
+ * enabled: entry `index` gives the kernel name, launches, total ms and its
+ * algorithmic bytes.  Returns the number of entries. */
+int mdas_kernel_stats(mdas_handle h, int index, char* name, int name_cap, int* launches,
+                      double* total_ms, double* bytes);
+
+/* Tunables of the reconstruction gather (tile of tile_w x tile_h pixels,
+ * tile_w*tile_h must be 64 or 16) and of the chunked leaf kernels. */
+int mdas_set_recon_tile(mdas_handle h, int tile_w, int tile_h);
+
+/* State transfer for tests and tools (synchronous).
+ * idx[L*4], box[L*2*dim] (per leaf: mn[dim] then mx[dim]), err[L], seed[L],
+ * coords[N*dim], values[N*3]; NULL pointers are skipped. */
+int mdas_download_state(mdas_handle h, float* coords, float* values, int* idx, float* box,
+                        float* err, uint32_t* seed);
+int mdas_upload_state(mdas_handle h, int n_samples, int n_leaves, const float* coords,
+                      const float* values, const int* idx, const float* box, const float* err,
+                      const uint32_t* seed, float max_error);
+int mdas_set_values(mdas_handle h, int offset, int count, const float* host_values);
+/* Sorted reconstruction keys (morton2(tile_x, tile_y) << 32 | leaf) of the
+ * last mdas_integrate; *n = number of keys. */
+int mdas_get_recon_keys(mdas_handle h, uint64_t* out, long long capacity, long long* n);
+
+const char* mdas_last_error(void);
+int mdas_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MDAS_H */

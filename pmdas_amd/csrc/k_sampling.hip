@@ -1,0 +1,539 @@
+// Sampling kernels of the MDAS grid: initial Morton grid, per-leaf error with
+// the global max, and the adaptive pass (selection, best-candidate placement,
+// median split) with indices allocated by a deterministic device-wide scan.
+//
+// Layout (HBM): sample coordinates AoS float[D] and values float3 (the
+// integrand contract); leaf slots int4; leaf boxes SoA by dimension so that
+// a wave reads one dimension of 64 consecutive leaves as 256 contiguous bytes;
+// errors, LCG states and a one-byte flag word per leaf.
+#include "mdas_device.hpp"
+#include "mdas_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace mdas {
+
+#define MDAS_ERR_CAPACITY_DEV (-2)
+
+// ----------------------------------------------------------------- initial --
+// kdgrid.cu:121-202.  One thread per initial leaf; four samples per leaf are
+// drawn from tea4(leaf, 0) in dimension order.  The 4*D coordinates of a
+// workgroup's 256 leaves are one contiguous range, written through LDS so
+// that every store instruction covers 256 contiguous bytes.
+template <int D>
+__global__ __launch_bounds__(256) void k_initial(DevGrid a, int L0) {
+    __shared__ float stage[256 * 4 * D];
+    const int cap = a.cap_l;
+    for (int base = blockIdx.x * 256; base < L0; base += gridDim.x * 256) {
+        const int i = base + threadIdx.x;
+        if (i < L0) {
+            float off[D], ext[D];
+            morton_cell<D>((unsigned)i, a.b, a.e, a.sX, a.sY, off, ext);
+            unsigned s = tea4((unsigned)i, 0u);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float r[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) r[d] = rnd(s);
+#pragma unroll
+                for (int d = 0; d < D; ++d) stage[threadIdx.x * 4 * D + j * D + d] = off[d] + r[d] * ext[d];
+            }
+            a.idx[i] = make_int4(4 * i, 4 * i + 1, 4 * i + 2, 4 * i + 3);
+            a.seed[i] = s;
+            a.err[i] = -1.0f;
+            a.flags[i] = 4;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                a.bmn[(size_t)d * cap + i] = off[d];
+                a.bmx[(size_t)d * cap + i] = off[d] + ext[d];
+            }
+        }
+        __syncthreads();
+        const int nl = min(256, L0 - base);
+        float* out = a.coords + (size_t)base * 4 * D;
+        for (int k = threadIdx.x; k < nl * 4 * D; k += 256) out[k] = stage[k];
+        __syncthreads();
+    }
+}
+
+// Counter set-up of InitialSampling (kdgrid.cu:639-642).
+__global__ void k_initial_counters(DevCounters* c, int L0) {
+    c->n_leaves = L0;
+    c->n_samples = 4 * L0;
+    c->n_leaves_old = 0;
+    c->n_samples_old = 0;
+    c->new_samples = 4 * L0;  // the first integrand launch covers all of them
+    c->new_nodes = 0;
+    c->gmax_bits = 0;
+    c->done = 0;
+    c->status = 0;
+    c->passes = 0;
+    c->limit = 0x7fffffff;
+    c->sel_cnt = 0;
+    c->stale = 0;
+    c->stale_vals = 0;
+}
+
+// ComputeErrors prologue (kdgrid.cu:684-685): g_error = 0.  A standalone call
+// also re-arms the pass kernels.
+__global__ void k_reset_errors(DevCounters* c, int standalone) {
+    if (standalone) c->done = 0;
+    c->gmax_bits = 0;
+}
+
+// One pass of the device-side loop (optixMotionBlur.cpp:781): stop when
+// N >= min(target, max_samples).
+__global__ void k_pass_begin(DevCounters* c) {
+    if (c->done) return;
+    if (c->n_samples >= c->limit) {
+        c->done = 1;
+        return;
+    }
+    c->gmax_bits = 0;
+}
+
+__global__ void k_set_limit(DevCounters* c, int limit) { c->limit = limit; }
+
+// ------------------------------------------------------------------ errors --
+// kdgrid.cu:204-288.  Leaves with error < 0 get the relative-L1 error of their
+// samples times their volume; every leaf takes part in the global max, which
+// is reduced per wave, per workgroup and then with one atomicMax on the float
+// bits (order-independent, so exact).
+template <int D>
+__global__ __launch_bounds__(256) void k_errors(DevGrid a) {
+    __shared__ float wmax[4];
+    __shared__ int wst[8];
+    if (a.cnt->done) return;
+    const int L = a.cnt->n_leaves;
+    const int cap = a.cap_l;
+    float m = 0.0f;
+    int n_stale = 0, n_vals = 0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < L; i += gridDim.x * 256) {
+        float e = a.err[i];
+        if (e < 0.0f) {
+            const int4 id = a.idx[i];
+            const int ids[4] = {id.x, id.y, id.z, id.w};
+            float vx[4] = {0.0f, 0.0f, 0.0f, 0.0f}, vy[4] = {0.0f, 0.0f, 0.0f, 0.0f},
+                  vz[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            int n = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (ids[q] >= 0) {
+                    const float* v = a.values + (size_t)ids[q] * 3;
+                    vx[q] = v[0];
+                    vy[q] = v[1];
+                    vz[q] = v[2];
+                    sx += vx[q];
+                    sy += vy[q];
+                    sz += vz[q];
+                    ++n;
+                }
+            }
+            // float3 /= float is a reciprocal multiply (sutil/vec_math.h:492-496)
+            const float inv = 1.0f / (float)n;
+            const float ax = sx * inv, ay = sy * inv, az = sz * inv;
+            float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (q < n) {
+                    dx += fabsf(vx[q] - ax);
+                    dy += fabsf(vy[q] - ay);
+                    dz += fabsf(vz[q] - az);
+                }
+            }
+            float vol = 1.0f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) vol *= (a.bmx[(size_t)d * cap + i] - a.bmn[(size_t)d * cap + i]);
+            e = 0.0f;
+            if (ax != 0.0f) e += dx / ax;
+            if (ay != 0.0f) e += dy / ay;
+            if (az != 0.0f) e += dz / az;
+            e /= (float)n;
+            e += 1.0e-5f;
+            e *= vol;
+            a.err[i] = e;
+            n_stale += 1;
+            n_vals += n;
+        }
+        m = fmaxf(m, e);
+    }
+    m = wave_max(m);
+    n_stale = wave_sum(n_stale);
+    n_vals = wave_sum(n_vals);
+    if ((threadIdx.x & 63) == 0) {
+        wmax[threadIdx.x >> 6] = m;
+        wst[threadIdx.x >> 6] = n_stale;
+        wst[4 + (threadIdx.x >> 6)] = n_vals;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float b = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+        atomicMax(&a.cnt->gmax_bits, __float_as_uint(b));
+        const int ts = wst[0] + wst[1] + wst[2] + wst[3];
+        const int tv = wst[4] + wst[5] + wst[6] + wst[7];
+        if (ts) {
+            atomicAdd(&a.cnt->stale, ts);
+            atomicAdd(&a.cnt->stale_vals, tv);
+        }
+    }
+}
+
+// --------------------------------------------------------------- selection --
+// kdgrid.cu:316-326.  Per leaf: the seed (tea4(i,0) when unset), one draw u,
+// selected iff pow(err / g_error, alpha) >= u.  A selected full leaf splits.
+// Writes the advanced LCG state and the flags, and one (selected, split)
+// count per workgroup for the allocation scan.
+__global__ __launch_bounds__(256) void k_select(DevGrid a) {
+    __shared__ int red[12];
+    if (a.cnt->done) return;
+    const int L = a.cnt->n_leaves;
+    const float gmax = __uint_as_float(a.cnt->gmax_bits);
+    const float alpha = a.alpha;
+    int beg, end;
+    chunk_range(L, gridDim.x, blockIdx.x, beg, end);
+    int ns = 0, nl = 0, nc = 0;
+    for (int i = beg + threadIdx.x; i < end; i += 256) {
+        const float e = a.err[i];
+        unsigned s = a.seed[i];
+        if (s == 0u) s = tea4((unsigned)i, 0u);
+        const float u = rnd(s);
+        const bool sel = powf_mdas(e / gmax, alpha) >= u;
+        const unsigned cnt = a.flags[i] & 7u;
+        const bool split = sel && cnt == 4u;
+        a.flags[i] = (unsigned char)(cnt | (sel ? 8u : 0u) | (split ? 16u : 0u));
+        a.seed[i] = s;
+        ns += sel ? 1 : 0;
+        nl += split ? 1 : 0;
+        nc += sel ? (int)cnt : 0;
+    }
+    ns = wave_sum(ns);
+    nl = wave_sum(nl);
+    nc = wave_sum(nc);
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = ns;
+        red[4 + (threadIdx.x >> 6)] = nl;
+        red[8 + (threadIdx.x >> 6)] = nc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a.part[blockIdx.x] = make_int2(red[0] + red[1] + red[2] + red[3], red[4] + red[5] + red[6] + red[7]);
+        const int tc = red[8] + red[9] + red[10] + red[11];
+        if (tc) atomicAdd(&a.cnt->sel_cnt, tc);
+    }
+}
+
+// Exclusive scan of the per-workgroup counts (one workgroup of 1024 threads),
+// then the capacity check and the counter update (kdgrid.cu:753-759).
+__global__ __launch_bounds__(1024) void k_scan_counts(DevGrid a, int G) {
+    __shared__ int ls[16], ll[16];
+    if (a.cnt->done) return;
+    const int per = (G + 1023) / 1024;
+    const int g0 = threadIdx.x * per;
+    int s = 0, l = 0;
+    for (int k = 0; k < per; ++k)
+        if (g0 + k < G) {
+            int2 p = a.part[g0 + k];
+            s += p.x;
+            l += p.y;
+        }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int xs = s, xl = l;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int ys = __shfl_up(xs, o, 64), yl = __shfl_up(xl, o, 64);
+        if (lane >= o) {
+            xs += ys;
+            xl += yl;
+        }
+    }
+    if (lane == 63) {
+        ls[w] = xs;
+        ll[w] = xl;
+    }
+    __syncthreads();
+    int ps = 0, pl = 0, ts = 0, tl = 0;
+    for (int k = 0; k < 16; ++k) {
+        if (k < w) {
+            ps += ls[k];
+            pl += ll[k];
+        }
+        ts += ls[k];
+        tl += ll[k];
+    }
+    int rs = ps + xs - s, rl = pl + xl - l;
+    for (int k = 0; k < per; ++k)
+        if (g0 + k < G) {
+            int2 p = a.part[g0 + k];
+            a.part_off[g0 + k] = make_int2(rs, rl);
+            rs += p.x;
+            rl += p.y;
+        }
+    if (threadIdx.x == 0) {
+        DevCounters* c = a.cnt;
+        const int N = c->n_samples, L = c->n_leaves;
+        if ((long long)N + ts > a.cap_s || (long long)L + tl > a.cap_l) {
+            c->status = MDAS_ERR_CAPACITY_DEV;
+            c->done = 1;
+        } else {
+            c->n_samples_old = N;
+            c->n_leaves_old = L;
+            c->n_samples = N + ts;
+            c->n_leaves = L + tl;
+            c->new_samples = ts;
+            c->new_nodes = tl;
+            const int p = c->passes;
+            if (p < kMaxPassStats) a.pstats[p] = PassStat{L, N, ts, tl, c->sel_cnt, c->stale, c->stale_vals, 0};
+            c->passes = p + 1;
+        }
+        c->sel_cnt = 0;
+        c->stale = 0;
+        c->stale_vals = 0;
+    }
+}
+
+// ----------------------------------------------------- placement and split --
+// kdgrid.cu:328-477 for one selected leaf: best candidate among
+// candidatesNum uniform points of the box (max over candidates of the min
+// Euclidean distance to the leaf's samples, earliest wins ties), appended at
+// sample index k; a full leaf is split at the median of its 5 samples along
+// the largest box axis, keeping the left half in place and the right half
+// at leaf index m.
+template <int D>
+__device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m, unsigned cnt) {
+    const int cap = a.cap_l;
+    unsigned s = a.seed[i];
+    float mn[D], mx[D], diag[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        mn[d] = a.bmn[(size_t)d * cap + i];
+        mx[d] = a.bmx[(size_t)d * cap + i];
+        diag[d] = mx[d] - mn[d];
+    }
+    const int4 id4 = a.idx[i];
+    const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
+    float p[4][D];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (ids[q] >= 0) {
+            const float* c = a.coords + (size_t)ids[q] * D;
+#pragma unroll
+            for (int d = 0; d < D; ++d) p[q][d] = c[d];
+        } else {
+#pragma unroll
+            for (int d = 0; d < D; ++d) p[q][d] = 0.0f;
+        }
+    }
+    float maxDist = -1.0f;
+    float best[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) best[d] = 0.0f;
+    for (int j = 0; j < a.cand; ++j) {
+        float c[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) c[d] = rnd(s);
+#pragma unroll
+        for (int d = 0; d < D; ++d) c[d] = mn[d] + c[d] * diag[d];
+        float minDist = 3.402823466e+38f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (ids[q] >= 0) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    float t = c[d] - p[q][d];
+                    acc += t * t;
+                }
+                const float dist = __builtin_sqrtf(acc);
+                if (minDist > dist) minDist = dist;
+            }
+        }
+        if (maxDist < minDist) {
+            maxDist = minDist;
+#pragma unroll
+            for (int d = 0; d < D; ++d) best[d] = c[d];
+        }
+    }
+    float* out = a.coords + (size_t)k * D;
+#pragma unroll
+    for (int d = 0; d < D; ++d) out[d] = best[d];
+
+    if (cnt < 4u) {
+        // kdgrid.cu:393-396: insert; the error is kept
+        int4 n4 = id4;
+        if (cnt == 0u) n4.x = k;
+        else if (cnt == 1u) n4.y = k;
+        else if (cnt == 2u) n4.z = k;
+        else n4.w = k;
+        a.idx[i] = n4;
+        a.flags[i] = (unsigned char)(cnt + 1u);
+    } else {
+        // kdgrid.cu:399-471: split.  LargestAxis scans from the last axis down
+        // with a strict '<' (aabb.h:34-45).
+        int axis = -1;
+        float ext = -3.402823466e+38f;
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d)
+            if (ext < diag[d]) {
+                ext = diag[d];
+                axis = d;
+            }
+        float key[5];
+        int v[5] = {ids[0], ids[1], ids[2], ids[3], k};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float kv = p[q][0];
+#pragma unroll
+            for (int d = 1; d < D; ++d)
+                if (d == axis) kv = p[q][d];
+            key[q] = kv;
+        }
+        {
+            float kv = best[0];
+#pragma unroll
+            for (int d = 1; d < D; ++d)
+                if (d == axis) kv = best[d];
+            key[4] = kv;
+        }
+        // stable bubble sort, strict '>' swaps (kdgrid.cu:409-417)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+#pragma unroll
+            for (int q = 0; q < 4 - r; ++q) {
+                if (key[q] > key[q + 1]) {
+                    float tk = key[q];
+                    key[q] = key[q + 1];
+                    key[q + 1] = tk;
+                    int tv = v[q];
+                    v[q] = v[q + 1];
+                    v[q + 1] = tv;
+                }
+            }
+        }
+        const float pos = 0.5f * (key[1] + key[2]);
+        const int E = ~D;
+        a.idx[i] = make_int4(v[0], v[1], E, E);
+        a.idx[m] = make_int4(v[2], v[3], v[4], E);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            a.bmn[(size_t)d * cap + m] = d == axis ? pos : mn[d];
+            a.bmx[(size_t)d * cap + m] = mx[d];
+        }
+        a.bmx[(size_t)axis * cap + i] = pos;
+        a.err[i] = -1.0f;
+        a.err[m] = -1.0f;
+        a.seed[m] = 0u;
+        a.flags[i] = 2;
+        a.flags[m] = 3;
+    }
+    a.seed[i] = s;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_place(DevGrid a) {
+    __shared__ int lds[8];
+    if (a.cnt->done) return;
+    const int L = a.cnt->n_leaves_old;
+    const int N0 = a.cnt->n_samples_old;
+    int beg, end;
+    chunk_range(L, gridDim.x, blockIdx.x, beg, end);
+    int2 base = a.part_off[blockIdx.x];
+    for (int t0 = beg; t0 < end; t0 += 256) {
+        const int i = t0 + threadIdx.x;
+        const unsigned f = i < end ? a.flags[i] : 0u;
+        const bool sel = (f & 8u) != 0u, split = (f & 16u) != 0u;
+        int rs, rl, ts, tl;
+        block_rank2(sel, split, lds, rs, rl, ts, tl);
+        if (sel) place_leaf<D>(a, i, N0 + base.x + rs, split ? L + base.y + rl : -1, f & 7u);
+        base.x += ts;
+        base.y += tl;
+    }
+}
+
+// --------------------------------------------------------------- fills --
+__global__ void k_fill_i32(int* p, int v, long long n) {
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256) p[i] = v;
+}
+__global__ void k_fill_f32(float* p, float v, long long n) {
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256) p[i] = v;
+}
+
+// ------------------------------------------------------------ launchers --
+static int grid_for(long long n, int cap_blocks = 8192) {
+    long long b = (n + 255) / 256;
+    if (b < 1) b = 1;
+    if (b > cap_blocks) b = cap_blocks;
+    return (int)b;
+}
+
+#define MDAS_DISPATCH_D(D_, STMT)                      \
+    switch (D_) {                                      \
+    case 3: { constexpr int DD = 3; STMT; } break;     \
+    case 4: { constexpr int DD = 4; STMT; } break;     \
+    case 5: { constexpr int DD = 5; STMT; } break;     \
+    case 6: { constexpr int DD = 6; STMT; } break;     \
+    default: return hipErrorInvalidValue;              \
+    }
+
+hipError_t launch_initial(Grid& g, int L0, hipStream_t st) {
+    const int blocks = grid_for(L0, 4096);
+    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_initial<DD>, dim3(blocks), dim3(256), 0, st, g.dev, L0));
+    hipLaunchKernelGGL(k_initial_counters, dim3(1), dim3(1), 0, st, g.dev.cnt, L0);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_errors(Grid& g, int standalone, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset_errors, dim3(1), dim3(1), 0, st, g.dev.cnt, standalone);
+    return hipGetLastError();
+}
+
+hipError_t launch_pass_begin(Grid& g, hipStream_t st) {
+    hipLaunchKernelGGL(k_pass_begin, dim3(1), dim3(1), 0, st, g.dev.cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_limit(Grid& g, int limit, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_limit, dim3(1), dim3(1), 0, st, g.dev.cnt, limit);
+    return hipGetLastError();
+}
+
+hipError_t launch_errors(Grid& g, hipStream_t st) {
+    // grid-stride over the device-side leaf count; sized for the capacity
+    const int blocks = grid_for(g.cap_l, 4096);
+    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_errors<DD>, dim3(blocks), dim3(256), 0, st, g.dev));
+    return hipGetLastError();
+}
+
+hipError_t launch_select(Grid& g, hipStream_t st) {
+    hipLaunchKernelGGL(k_select, dim3(kChunkBlocks), dim3(256), 0, st, g.dev);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(Grid& g, hipStream_t st) {
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, g.dev, kChunkBlocks);
+    return hipGetLastError();
+}
+
+hipError_t launch_place(Grid& g, hipStream_t st) {
+    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_place<DD>, dim3(kChunkBlocks), dim3(256), 0, st, g.dev));
+    return hipGetLastError();
+}
+
+hipError_t launch_adaptive(Grid& g, hipStream_t st) {
+    hipError_t e = launch_select(g, st);
+    if (e == hipSuccess) e = launch_scan(g, st);
+    if (e == hipSuccess) e = launch_place(g, st);
+    return e;
+}
+
+hipError_t launch_fill_i32(int* p, int v, long long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n)), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(float* p, float v, long long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n)), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+}  // namespace mdas

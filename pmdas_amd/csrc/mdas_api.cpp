@@ -1,0 +1,794 @@
+// C ABI of libmdas (include/mdas.h): handle lifetime, the reference's KDGrid
+// entry points, the device-side sampling loop, timing and state transfer.
+#include "mdas.h"
+
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "mdas_internal.hpp"
+#include "scenes.hpp"
+
+using mdas::Grid;
+
+struct mdas_grid {
+    Grid g;
+};
+
+struct mdas_scene {
+    mscene::Params host;
+    void* dev = nullptr;
+};
+
+namespace {
+
+thread_local std::string t_err;
+
+int fail(int code, const std::string& msg) {
+    t_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(MDAS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+hipStream_t S(void* s) { return (hipStream_t)s; }
+
+template <typename T>
+hipError_t dalloc(T** p, size_t n) {
+    return hipMalloc((void**)p, sizeof(T) * (n ? n : 1));
+}
+
+void free_grid(Grid& g) {
+    mdas::DevGrid& d = g.dev;
+    void* ptrs[] = {d.coords, d.values, d.idx,  d.bmn,    d.bmx,    d.err,         d.seed,
+                    d.flags,  d.part,   d.part_off, d.cnt, d.pstats, g.keys, g.keys_alt,
+                    g.tile_start, g.tile_end, g.rs_hist, g.d_scalar};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    d = mdas::DevGrid{};
+    g.keys = g.keys_alt = nullptr;
+    g.tile_start = g.tile_end = nullptr;
+    g.rs_hist = nullptr;
+    g.d_scalar = nullptr;
+}
+
+int sync_counters(Grid& g, hipStream_t st) {
+    HIP_TRY(hipMemcpyAsync(&g.host, g.dev.cnt, sizeof(mdas::DevCounters), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    g.gmax_host = 0.0f;
+    memcpy(&g.gmax_host, &g.host.gmax_bits, 4);
+    return MDAS_OK;
+}
+
+void log_kv(Grid& g, const char* key, double v) {
+    if (g.log_path.empty()) return;
+    std::ofstream f(g.log_path, std::ios::app);
+    f << key << "\n" << v << "\n";
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0f;
+    return ms;
+}
+
+// Fold the recorded kernel events into per-kernel totals and phase times.
+void collect_kernel_times(Grid& g) {
+    if (g.krecs.empty()) return;
+    if (g.krecs.back().b) (void)hipEventSynchronize(g.krecs.back().b);
+    for (const auto& r : g.krecs) {
+        const double ms = (r.a && r.b) ? elapsed(r.a, r.b) : 0.0;
+        bool found = false;
+        for (auto& k : g.kstats)
+            if (k.name == r.name) {
+                k.launches++;
+                k.ms += ms;
+                found = true;
+                break;
+            }
+        if (!found) g.kstats.push_back(Grid::KStat{r.name, 1, ms});
+    }
+    g.krecs.clear();
+    g.evnext = 0;
+    auto get = [&](const char* n) {
+        for (auto& k : g.kstats)
+            if (k.name == n) return k.ms;
+        return 0.0;
+    };
+    g.t_initial = get("k_initial");
+    g.t_errors = get("k_errors");
+    g.t_adaptive = get("k_select") + get("k_scan_counts") + get("k_place");
+    g.t_integrand = get("k_eval");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mdas_last_error(void) { return t_err.c_str(); }
+
+int mdas_device_synchronize(void) {
+    HIP_TRY(hipDeviceSynchronize());
+    return MDAS_OK;
+}
+
+int mdas_create(const mdas_config* cfg, mdas_handle* out) {
+    if (!cfg || !out) return fail(MDAS_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const int D = cfg->dim;
+    if (D < 3 || D > 6) return fail(MDAS_ERR_INVALID, "dim must be 3..6");
+    if (cfg->bits_per_dim < 0 || cfg->extra_img_bits < 0)
+        return fail(MDAS_ERR_INVALID, "bitsPerDim and extraImgBits must be >= 0");
+    if (cfg->candidates_num < 1) return fail(MDAS_ERR_INVALID, "candidatesNum must be >= 1");
+    if (!(cfg->alpha >= 0.0f)) return fail(MDAS_ERR_INVALID, "alpha must be >= 0");
+    if (!(cfg->scale_x > 0.0f) || !(cfg->scale_y > 0.0f))
+        return fail(MDAS_ERR_INVALID, "scale_x and scale_y must be > 0");
+    if (cfg->max_samples < 8 || cfg->max_samples > 0x7fffffffLL)
+        return fail(MDAS_ERR_INVALID, "max_samples must be in [8, 2^31-1]");
+    const int bits = cfg->bits_per_dim * D + 2 * cfg->extra_img_bits;
+    if (bits > 28) return fail(MDAS_ERR_INVALID, "initial grid too large (bitsPerDim*dim + 2*extraImgBits > 28)");
+    const long long L0 = 1LL << bits;
+    if (4 * L0 > cfg->max_samples)
+        return fail(MDAS_ERR_INVALID, "max_samples is smaller than the initial sample count 4*2^(b*dim+2e)");
+
+    mdas_grid* h = new mdas_grid;
+    Grid& g = h->g;
+    g.D = D;
+    g.cand = cfg->candidates_num;
+    g.b = cfg->bits_per_dim;
+    g.e = cfg->extra_img_bits;
+    g.alpha = cfg->alpha;
+    g.sX = cfg->scale_x;
+    g.sY = cfg->scale_y;
+    g.max_samples = cfg->max_samples;
+    g.cap_s = (int)cfg->max_samples;
+    g.cap_l = g.cap_s / 2 + 1;  // every leaf holds >= 2 samples, so L <= N/2
+    if (cfg->log_path) g.log_path = cfg->log_path;
+    mdas::DevGrid& d = g.dev;
+    hipError_t e = hipSuccess;
+    const size_t cs = (size_t)g.cap_s, cl = (size_t)g.cap_l;
+    if (e == hipSuccess) e = dalloc(&d.coords, cs * D);
+    if (e == hipSuccess) e = dalloc(&d.values, cs * 3);
+    if (e == hipSuccess) e = dalloc(&d.idx, cl);
+    if (e == hipSuccess) e = dalloc(&d.bmn, cl * D);
+    if (e == hipSuccess) e = dalloc(&d.bmx, cl * D);
+    if (e == hipSuccess) e = dalloc(&d.err, cl);
+    if (e == hipSuccess) e = dalloc(&d.seed, cl);
+    if (e == hipSuccess) e = dalloc(&d.flags, cl);
+    if (e == hipSuccess) e = dalloc(&d.part, (size_t)mdas::kChunkBlocks);
+    if (e == hipSuccess) e = dalloc(&d.part_off, (size_t)mdas::kChunkBlocks);
+    if (e == hipSuccess) e = dalloc(&d.cnt, 1);
+    if (e == hipSuccess) e = dalloc(&d.pstats, (size_t)mdas::kMaxPassStats);
+    if (e == hipSuccess) e = dalloc(&g.d_scalar, 64);
+    if (e != hipSuccess) {
+        free_grid(g);
+        delete h;
+        return fail(MDAS_ERR_HIP, std::string("device allocation failed: ") + hipGetErrorString(e));
+    }
+    e = hipMemset(d.cnt, 0, sizeof(mdas::DevCounters));
+    if (e == hipSuccess) e = hipMemset(d.values, 0, sizeof(float) * cs * 3);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        free_grid(g);
+        delete h;
+        return fail(MDAS_ERR_HIP, std::string("device init failed: ") + hipGetErrorString(e));
+    }
+    d.cap_s = g.cap_s;
+    d.cap_l = g.cap_l;
+    d.cand = g.cand;
+    d.b = g.b;
+    d.e = g.e;
+    d.alpha = g.alpha;
+    d.sX = g.sX;
+    d.sY = g.sY;
+    if (!g.log_path.empty()) std::ofstream(g.log_path, std::ios::trunc);
+    *out = h;
+    return MDAS_OK;
+}
+
+int mdas_destroy(mdas_handle h) {
+    if (!h) return MDAS_OK;
+    hipDeviceSynchronize();
+    free_grid(h->g);
+    delete h;
+    return MDAS_OK;
+}
+
+int mdas_initial_sampling(mdas_handle h, void* stream) {
+    if (!h) return fail(MDAS_ERR_INVALID, "null handle");
+    Grid& g = h->g;
+    hipStream_t st = S(stream);
+    const int L0 = 1 << (g.b * g.D + 2 * g.e);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool logging = !g.log_path.empty();
+    if (logging) {
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, st));
+    }
+    // seeds of leaves that do not exist yet must read as 0 (kdgrid.cu:637)
+    HIP_TRY(mdas::launch_fill_i32((int*)g.dev.seed, 0, g.cap_l, st));
+    HIP_TRY(mdas::launch_initial(g, L0, st));
+    g.initialized = true;
+    g.host.n_leaves = L0;
+    g.host.n_samples = 4 * L0;
+    g.host.new_samples = 4 * L0;
+    g.host.new_nodes = 0;
+    g.host.passes = 0;
+    if (logging) {
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        log_kv(g, "INITIAL SAMPLES", 4.0 * L0);
+        log_kv(g, "INITIAL SAMPLING TIME", elapsed(e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    return MDAS_OK;
+}
+
+int mdas_compute_errors(mdas_handle h, void* stream) {
+    if (!h) return fail(MDAS_ERR_INVALID, "null handle");
+    Grid& g = h->g;
+    if (!g.initialized) return fail(MDAS_ERR_STATE, "InitialSampling has not been called");
+    hipStream_t st = S(stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool logging = !g.log_path.empty();
+    if (logging) {
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, st));
+    }
+    HIP_TRY(mdas::launch_reset_errors(g, 1, st));
+    HIP_TRY(mdas::launch_errors(g, st));
+    if (logging) {
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        log_kv(g, "COMPUTE ERRORS TIME", elapsed(e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    return MDAS_OK;
+}
+
+int mdas_adaptive_sampling(mdas_handle h, void* stream) {
+    if (!h) return fail(MDAS_ERR_INVALID, "null handle");
+    Grid& g = h->g;
+    if (!g.initialized) return fail(MDAS_ERR_STATE, "InitialSampling has not been called");
+    hipStream_t st = S(stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool logging = !g.log_path.empty();
+    if (logging) {
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, st));
+    }
+    HIP_TRY(mdas::launch_adaptive(g, st));
+    int rc = sync_counters(g, st);  // kdgrid.cu:753-759 reads the counters back
+    if (rc) return rc;
+    if (g.host.status != 0) {
+        int s = g.host.status;
+        // clear the device error so that the grid stays usable
+        mdas::DevCounters c = g.host;
+        c.status = 0;
+        c.done = 0;
+        HIP_TRY(hipMemcpy(g.dev.cnt, &c, sizeof(c), hipMemcpyHostToDevice));
+        g.host = c;
+        if (s == MDAS_ERR_CAPACITY)
+            return fail(MDAS_ERR_CAPACITY, "adaptive pass would exceed max_samples (" +
+                                               std::to_string(g.max_samples) + ")");
+        return fail(s, "device error");
+    }
+    if (logging) {
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        log_kv(g, "SAMPLES", g.host.new_samples);
+        log_kv(g, "ADAPTIVE SAMPLING TIME", elapsed(e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    return MDAS_OK;
+}
+
+int mdas_sampling_pass(mdas_handle h, void* stream) {
+    int rc = mdas_compute_errors(h, stream);
+    if (rc) return rc;
+    return mdas_adaptive_sampling(h, stream);
+}
+
+static int check_image(Grid& g, int w, int hgt) {
+    if (w <= 0 || hgt <= 0) return fail(MDAS_ERR_INVALID, "image size must be positive");
+    if ((long long)w * hgt > 0x7fffffffLL) return fail(MDAS_ERR_INVALID, "image too large");
+    if (!g.initialized) return fail(MDAS_ERR_STATE, "InitialSampling has not been called");
+    return MDAS_OK;
+}
+
+int mdas_integrate(mdas_handle h, float* pixels, int w, int hgt, void* stream) {
+    if (!h || !pixels) return fail(MDAS_ERR_INVALID, "null argument");
+    Grid& g = h->g;
+    int rc = check_image(g, w, hgt);
+    if (rc) return rc;
+    hipStream_t st = S(stream);
+    const bool logging = !g.log_path.empty();
+    const bool ev = g.timing || logging;
+    hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (ev) {
+        for (auto& x : e) HIP_TRY(hipEventCreate(&x));
+        HIP_TRY(hipEventRecord(e[0], st));
+    }
+    HIP_TRY(mdas::recon_prepare(g, w, hgt, st));
+    HIP_TRY(mdas::launch_integrate_gather(g, pixels, w, hgt, st, e[1], e[2]));
+    if (ev) {
+        HIP_TRY(hipEventRecord(e[3], st));
+        HIP_TRY(hipEventSynchronize(e[3]));
+        g.t_recon = elapsed(e[0], e[3]);
+        g.t_sort = elapsed(e[1], e[2]);
+        for (auto& x : e) (void)hipEventDestroy(x);
+        collect_kernel_times(g);
+    }
+    if (logging) {
+        int rc2 = sync_counters(g, st);
+        if (rc2) return rc2;
+        log_kv(g, "TOTAL SAMPLES", g.host.n_samples);
+        log_kv(g, "CONSTRUCT TIME", g.t_sort);
+        log_kv(g, "INTEGRATE TIME", g.t_recon);
+    }
+    return MDAS_OK;
+}
+
+int mdas_integrate_scatter(mdas_handle h, float* pixels, int w, int hgt, void* stream) {
+    if (!h || !pixels) return fail(MDAS_ERR_INVALID, "null argument");
+    Grid& g = h->g;
+    int rc = check_image(g, w, hgt);
+    if (rc) return rc;
+    HIP_TRY(mdas::launch_integrate_scatter(g, pixels, w, hgt, S(stream)));
+    return MDAS_OK;
+}
+
+int mdas_convert_to_bytes(mdas_handle h, const float* pixels, unsigned char* bytes, int w, int hgt,
+                          void* stream) {
+    if (!pixels || !bytes) return fail(MDAS_ERR_INVALID, "null argument");
+    if (w <= 0 || hgt <= 0) return fail(MDAS_ERR_INVALID, "image size must be positive");
+    (void)h;
+    HIP_TRY(mdas::launch_bytes(pixels, bytes, (long long)w * hgt, S(stream)));
+    return MDAS_OK;
+}
+
+int mdas_sampling_density(mdas_handle h, float* host_pixels, int w, int hgt) {
+    if (!h || !host_pixels) return fail(MDAS_ERR_INVALID, "null argument");
+    Grid& g = h->g;
+    int rc = check_image(g, w, hgt);
+    if (rc) return rc;
+    rc = sync_counters(g, nullptr);
+    if (rc) return rc;
+    const long long N = g.host.n_samples;
+    std::vector<float> c((size_t)N * g.D);
+    HIP_TRY(hipMemcpy(c.data(), g.dev.coords, sizeof(float) * c.size(), hipMemcpyDeviceToHost));
+    memset(host_pixels, 0, sizeof(float) * 4 * (size_t)w * hgt);
+    const float dens = 0.05f;
+    for (long long i = 0; i < N; ++i) {
+        int x = (int)(c[(size_t)i * g.D] / g.sX * (float)w);
+        int y = (int)(c[(size_t)i * g.D + 1] / g.sY * (float)hgt);
+        x = std::min(x, w - 1);
+        y = std::min(y, hgt - 1);
+        float* p = host_pixels + 4 * ((size_t)y * w + x);
+        for (int k = 0; k < 4; ++k) p[k] += dens;
+    }
+    for (long long i = 0; i < (long long)w * hgt; ++i) host_pixels[4 * i + 3] = 1.0f;
+    return MDAS_OK;
+}
+
+int mdas_download_state(mdas_handle h, float* coords, float* values, int* idx, float* box,
+                        float* err, uint32_t* seed) {
+    if (!h) return fail(MDAS_ERR_INVALID, "null handle");
+    Grid& g = h->g;
+    int rc = sync_counters(g, nullptr);
+    if (rc) return rc;
+    const size_t N = (size_t)g.host.n_samples, L = (size_t)g.host.n_leaves;
+    const int D = g.D;
+    if (coords) HIP_TRY(hipMemcpy(coords, g.dev.coords, sizeof(float) * N * D, hipMemcpyDeviceToHost));
+    if (values) HIP_TRY(hipMemcpy(values, g.dev.values, sizeof(float) * N * 3, hipMemcpyDeviceToHost));
+    if (idx) HIP_TRY(hipMemcpy(idx, g.dev.idx, sizeof(int) * 4 * L, hipMemcpyDeviceToHost));
+    if (err) HIP_TRY(hipMemcpy(err, g.dev.err, sizeof(float) * L, hipMemcpyDeviceToHost));
+    if (seed) HIP_TRY(hipMemcpy(seed, g.dev.seed, sizeof(uint32_t) * L, hipMemcpyDeviceToHost));
+    if (box) {
+        std::vector<float> mn(L), mx(L);
+        for (int d = 0; d < D; ++d) {
+            HIP_TRY(hipMemcpy(mn.data(), g.dev.bmn + (size_t)d * g.cap_l, sizeof(float) * L, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(mx.data(), g.dev.bmx + (size_t)d * g.cap_l, sizeof(float) * L, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < L; ++i) {
+                box[i * 2 * D + d] = mn[i];
+                box[i * 2 * D + D + d] = mx[i];
+            }
+        }
+    }
+    return MDAS_OK;
+}
+
+int mdas_upload_state(mdas_handle h, int n, int l, const float* coords, const float* values,
+                      const int* idx, const float* box, const float* err, const uint32_t* seed,
+                      float max_error) {
+    if (!h) return fail(MDAS_ERR_INVALID, "null handle");
+    Grid& g = h->g;
+    if (n < 0 || l < 0 || n > g.cap_s || l > g.cap_l) return fail(MDAS_ERR_INVALID, "state exceeds capacity");
+    const int D = g.D;
+    HIP_TRY(hipDeviceSynchronize());
+    if (coords) HIP_TRY(hipMemcpy(g.dev.coords, coords, sizeof(float) * (size_t)n * D, hipMemcpyHostToDevice));
+    if (values) HIP_TRY(hipMemcpy(g.dev.values, values, sizeof(float) * (size_t)n * 3, hipMemcpyHostToDevice));
+    if (err) HIP_TRY(hipMemcpy(g.dev.err, err, sizeof(float) * (size_t)l, hipMemcpyHostToDevice));
+    if (seed) {
+        std::vector<uint32_t> s((size_t)g.cap_l, 0u);
+        memcpy(s.data(), seed, sizeof(uint32_t) * (size_t)l);
+        HIP_TRY(hipMemcpy(g.dev.seed, s.data(), sizeof(uint32_t) * s.size(), hipMemcpyHostToDevice));
+    }
+    if (idx) {
+        HIP_TRY(hipMemcpy(g.dev.idx, idx, sizeof(int) * 4 * (size_t)l, hipMemcpyHostToDevice));
+        std::vector<unsigned char> f((size_t)l);
+        for (int i = 0; i < l; ++i) {
+            int c = 0;
+            for (int q = 0; q < 4; ++q) c += idx[4 * i + q] >= 0 ? 1 : 0;
+            f[i] = (unsigned char)c;
+        }
+        HIP_TRY(hipMemcpy(g.dev.flags, f.data(), f.size(), hipMemcpyHostToDevice));
+    }
+    if (box) {
+        std::vector<float> mn((size_t)l), mx((size_t)l);
+        for (int d = 0; d < D; ++d) {
+            for (int i = 0; i < l; ++i) {
+                mn[i] = box[(size_t)i * 2 * D + d];
+                mx[i] = box[(size_t)i * 2 * D + D + d];
+            }
+            HIP_TRY(hipMemcpy(g.dev.bmn + (size_t)d * g.cap_l, mn.data(), sizeof(float) * l, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(g.dev.bmx + (size_t)d * g.cap_l, mx.data(), sizeof(float) * l, hipMemcpyHostToDevice));
+        }
+    }
+    mdas::DevCounters c{};
+    c.n_samples = n;
+    c.n_leaves = l;
+    c.limit = 0x7fffffff;
+    memcpy(&c.gmax_bits, &max_error, 4);
+    HIP_TRY(hipMemcpy(g.dev.cnt, &c, sizeof(c), hipMemcpyHostToDevice));
+    g.host = c;
+    g.initialized = true;
+    return MDAS_OK;
+}
+
+int mdas_set_values(mdas_handle h, int offset, int count, const float* hv) {
+    if (!h || !hv) return fail(MDAS_ERR_INVALID, "null argument");
+    Grid& g = h->g;
+    if (offset < 0 || count < 0 || (long long)offset + count > g.cap_s)
+        return fail(MDAS_ERR_INVALID, "range exceeds capacity");
+    HIP_TRY(hipMemcpy(g.dev.values + (size_t)offset * 3, hv, sizeof(float) * 3 * (size_t)count,
+                      hipMemcpyHostToDevice));
+    return MDAS_OK;
+}
+
+int mdas_get_recon_keys(mdas_handle h, uint64_t* out, long long capacity, long long* n) {
+    if (!h) return fail(MDAS_ERR_INVALID, "null handle");
+    Grid& g = h->g;
+    HIP_TRY(hipDeviceSynchronize());
+    if (n) *n = g.last_n_keys;
+    if (out && g.keys) {
+        long long m = std::min(capacity, g.last_n_keys);
+        HIP_TRY(hipMemcpy(out, g.keys, sizeof(uint64_t) * (size_t)m, hipMemcpyDeviceToHost));
+    }
+    return MDAS_OK;
+}
+
+int mdas_validate(mdas_handle h, int* valid) {
+    if (!h || !valid) return fail(MDAS_ERR_INVALID, "null argument");
+    Grid& g = h->g;
+    *valid = 0;
+    int rc = sync_counters(g, nullptr);
+    if (rc) return rc;
+    const int D = g.D;
+    const long long N = g.host.n_samples, L = g.host.n_leaves;
+    std::vector<float> coords((size_t)N * D), box((size_t)L * 2 * D);
+    std::vector<int> idx((size_t)L * 4);
+    rc = mdas_download_state(h, coords.data(), nullptr, idx.data(), box.data(), nullptr, nullptr);
+    if (rc) return rc;
+    bool ok = true;
+    std::vector<int> hist((size_t)N, 0);
+    double vol = 0.0;
+    long long total = 0;
+    for (long long i = 0; i < L && ok; ++i) {
+        const float* mn = &box[(size_t)i * 2 * D];
+        const float* mx = mn + D;
+        int n = 0;
+        bool prefix = true;
+        for (int q = 0; q < 4; ++q) {
+            int k = idx[(size_t)i * 4 + q];
+            if (k >= 0) {
+                if (!prefix || k >= N) { ok = false; break; }
+                for (int d = 0; d < D; ++d) {  // aabb.h Contains
+                    float c = coords[(size_t)k * D + d];
+                    if (c < mn[d] || c > mx[d]) ok = false;
+                }
+                hist[k]++;
+                ++n;
+            } else {
+                if (k != ~D) ok = false;
+                prefix = false;
+            }
+        }
+        if (n < 2) ok = false;
+        double v = 1.0;
+        for (int d = 0; d < D; ++d) {
+            if (!(mn[d] < mx[d])) ok = false;
+            v *= (double)(mx[d] - mn[d]);
+        }
+        vol += v;
+        total += n;
+    }
+    const double root = (double)g.sX * (double)g.sY;
+    if (std::fabs(vol - root) > 1.0e-2 * root) ok = false;  // kdgrid.cu:909-914
+    if (total != N) ok = false;                              // kdgrid.cu:916-920
+    for (long long k = 0; k < N && ok; ++k)
+        if (hist[k] != 1) ok = false;                        // kdgrid.cu:922-956
+    *valid = ok ? 1 : 0;
+    return MDAS_OK;
+}
+
+float* mdas_get_sample_coordinates(mdas_handle h) { return h ? h->g.dev.coords : nullptr; }
+float* mdas_get_sample_values(mdas_handle h) { return h ? h->g.dev.values : nullptr; }
+int mdas_get_dim(mdas_handle h) { return h ? h->g.D : 0; }
+int mdas_get_candidates_num(mdas_handle h) { return h ? h->g.cand : 0; }
+int mdas_get_max_node_size(mdas_handle h) { return h ? 4 : 0; }
+int mdas_get_new_samples(mdas_handle h) { return h ? h->g.host.new_samples : 0; }
+int mdas_get_new_nodes(mdas_handle h) { return h ? h->g.host.new_nodes : 0; }
+int mdas_get_number_of_samples(mdas_handle h) { return h ? h->g.host.n_samples : 0; }
+int mdas_get_number_of_nodes(mdas_handle h) { return h ? h->g.host.n_leaves : 0; }
+int mdas_get_max_samples(mdas_handle h) { return h ? (int)h->g.max_samples : 0; }
+float mdas_get_scale_x(mdas_handle h) { return h ? h->g.sX : 0.0f; }
+float mdas_get_scale_y(mdas_handle h) { return h ? h->g.sY : 0.0f; }
+float mdas_get_max_error(mdas_handle h) {
+    if (!h) return 0.0f;
+    if (sync_counters(h->g, nullptr)) return 0.0f;
+    return h->g.gmax_host;
+}
+int mdas_get_passes(mdas_handle h) { return h ? h->g.passes : 0; }
+
+// ------------------------------------------------------------------ scenes --
+int mdas_scene_create(int kind, uint32_t seed, int dim, float sx, float sy, mdas_scene_handle* out) {
+    if (!out) return fail(MDAS_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (kind < 0 || kind > 5) return fail(MDAS_ERR_INVALID, "unknown scene kind");
+    static const int need[6] = {2, 3, 4, 5, 6, 2};
+    if (dim < need[kind] || dim > 6) return fail(MDAS_ERR_INVALID, "scene needs more dimensions");
+    mdas_scene* s = new mdas_scene;
+    mscene::make_scene(&s->host, kind, seed, dim, sx, sy);
+    hipError_t e = hipMalloc(&s->dev, sizeof(mscene::Params));
+    if (e == hipSuccess) e = hipMemcpy(s->dev, &s->host, sizeof(mscene::Params), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (s->dev) hipFree(s->dev);
+        delete s;
+        return fail(MDAS_ERR_HIP, std::string("scene upload failed: ") + hipGetErrorString(e));
+    }
+    *out = s;
+    return MDAS_OK;
+}
+
+int mdas_scene_destroy(mdas_scene_handle s) {
+    if (!s) return MDAS_OK;
+    hipDeviceSynchronize();
+    if (s->dev) hipFree(s->dev);
+    delete s;
+    return MDAS_OK;
+}
+
+long long mdas_scene_params(mdas_scene_handle s, void* out, long long cap) {
+    if (!s) return 0;
+    const long long n = (long long)sizeof(mscene::Params);
+    if (out && cap >= n) memcpy(out, &s->host, (size_t)n);
+    return n;
+}
+
+int mdas_eval_integrand(mdas_handle h, mdas_scene_handle s, int offset, int count, void* stream) {
+    if (!h || !s) return fail(MDAS_ERR_INVALID, "null argument");
+    Grid& g = h->g;
+    if (s->host.dim != g.D) return fail(MDAS_ERR_INVALID, "scene and grid dimensions differ");
+    if (offset < 0 || count < 0 || (long long)offset + count > g.cap_s)
+        return fail(MDAS_ERR_INVALID, "sample range exceeds capacity");
+    if (count == 0) return MDAS_OK;
+    HIP_TRY(mdas::launch_eval(g, s->dev, offset, count, false, S(stream)));
+    return MDAS_OK;
+}
+
+int mdas_eval_new_samples(mdas_handle h, mdas_scene_handle s, void* stream) {
+    if (!h || !s) return fail(MDAS_ERR_INVALID, "null argument");
+    Grid& g = h->g;
+    if (s->host.dim != g.D) return fail(MDAS_ERR_INVALID, "scene and grid dimensions differ");
+    HIP_TRY(mdas::launch_eval(g, s->dev, 0, 0, true, S(stream)));
+    return MDAS_OK;
+}
+
+// ---------------------------------------------------------------- the loop --
+int mdas_set_timing(mdas_handle h, int enable) {
+    if (!h) return fail(MDAS_ERR_INVALID, "null handle");
+    h->g.timing = enable != 0;
+    return MDAS_OK;
+}
+
+int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream) {
+    if (!h || !s) return fail(MDAS_ERR_INVALID, "null argument");
+    Grid& g = h->g;
+    if (s->host.dim != g.D) return fail(MDAS_ERR_INVALID, "scene and grid dimensions differ");
+    if (target < 0) return fail(MDAS_ERR_INVALID, "target must be >= 0");
+    hipStream_t st = S(stream);
+    const int L0 = 1 << (g.b * g.D + 2 * g.e);
+    const long long limit = std::min<long long>(target, g.max_samples);
+    const int B = std::max(1, g.passes_per_sync);
+    g.kstats.clear();
+    g.krecs.clear();
+    g.evnext = 0;
+    g.t_recon = g.t_sort = 0.0;
+    mdas::kbeg(g, st, "k_initial");
+    HIP_TRY(mdas::launch_fill_i32((int*)g.dev.seed, 0, g.cap_l, st));
+    HIP_TRY(mdas::launch_initial(g, L0, st));
+    HIP_TRY(mdas::launch_set_limit(g, (int)limit, st));
+    mdas::kend(g, st);
+    g.initialized = true;
+    mdas::kbeg(g, st, "k_eval");
+    HIP_TRY(mdas::launch_eval(g, s->dev, 0, 4 * L0, false, st));
+    mdas::kend(g, st);
+    int enqueued = 0;
+    for (;;) {
+        for (int b = 0; b < B; ++b) {
+            mdas::kbeg(g, st, "k_errors");
+            HIP_TRY(mdas::launch_pass_begin(g, st));
+            HIP_TRY(mdas::launch_errors(g, st));
+            mdas::kend(g, st);
+            mdas::kbeg(g, st, "k_select");
+            HIP_TRY(mdas::launch_select(g, st));
+            mdas::kend(g, st);
+            mdas::kbeg(g, st, "k_scan_counts");
+            HIP_TRY(mdas::launch_scan(g, st));
+            mdas::kend(g, st);
+            mdas::kbeg(g, st, "k_place");
+            HIP_TRY(mdas::launch_place(g, st));
+            mdas::kend(g, st);
+            mdas::kbeg(g, st, "k_eval");
+            HIP_TRY(mdas::launch_eval(g, s->dev, 0, 0, true, st));
+            mdas::kend(g, st);
+            ++enqueued;
+        }
+        int rc = sync_counters(g, st);
+        if (rc) return rc;
+        if (g.host.done || g.host.status) break;
+        if (enqueued > 100000) return fail(MDAS_ERR_STATE, "sampling loop did not terminate");
+    }
+    g.passes = g.host.passes;
+    collect_kernel_times(g);
+    g.pstats_host.assign((size_t)std::min(g.passes, mdas::kMaxPassStats), mdas::PassStat{});
+    if (!g.pstats_host.empty())
+        HIP_TRY(hipMemcpy(g.pstats_host.data(), g.dev.pstats, sizeof(mdas::PassStat) * g.pstats_host.size(),
+                          hipMemcpyDeviceToHost));
+    if (!g.log_path.empty()) {
+        log_kv(g, "INITIAL SAMPLES", 4.0 * L0);
+        log_kv(g, "INITIAL SAMPLING TIME", g.t_initial);
+        for (auto& p : g.pstats_host) log_kv(g, "SAMPLES", p.sel);
+        log_kv(g, "COMPUTE ERRORS TIME", g.t_errors);
+        log_kv(g, "ADAPTIVE SAMPLING TIME", g.t_adaptive);
+        log_kv(g, "TRACE TIME", g.t_integrand);
+    }
+    if (g.host.status) {
+        int s2 = g.host.status;
+        return fail(s2, s2 == MDAS_ERR_CAPACITY ? "adaptive pass would exceed max_samples" : "device error");
+    }
+    return MDAS_OK;
+}
+
+int mdas_get_pass_stats(mdas_handle h, int* out, int max_rows) {
+    if (!h || !out) return fail(MDAS_ERR_INVALID, "null argument");
+    Grid& g = h->g;
+    int n = std::min<int>((int)g.pstats_host.size(), max_rows);
+    for (int i = 0; i < n; ++i) {
+        const mdas::PassStat& p = g.pstats_host[i];
+        const int row[8] = {p.L, p.N, p.sel, p.split, p.selcnt, p.stale, p.stale_vals, 0};
+        memcpy(out + 8 * i, row, sizeof(row));
+    }
+    return n;
+}
+
+// Algorithmic bytes of each kernel over the last run + reconstruction: the
+// bytes the algorithm must move with this layout (FP32 / int32 values, int4
+// slots, SoA boxes of 8*D bytes, one flag byte), counted once.
+static double kernel_bytes(const Grid& g, const std::string& k) {
+    const double D = g.D;
+    const double L0 = (double)(1LL << (g.b * g.D + 2 * g.e));
+    double b = 0.0;
+    if (k == "k_initial") return L0 * (16.0 * D + 16.0 + 8.0 * D + 4.0 + 4.0 + 1.0) + 4.0 * g.cap_l;
+    if (k == "k_eval") {
+        double n = (double)g.host.n_samples;
+        return n * (4.0 * D + 12.0);
+    }
+    for (const auto& p : g.pstats_host) {
+        if (k == "k_errors")
+            b += 4.0 * p.L + (double)p.stale * (16.0 + 8.0 * D + 4.0) + 12.0 * p.stale_vals;
+        else if (k == "k_select")
+            b += 14.0 * p.L;
+        else if (k == "k_scan_counts")
+            b += 2.0 * 8.0 * mdas::kChunkBlocks;
+        else if (k == "k_place")
+            b += 1.0 * p.L +
+                 (double)p.sel * (4.0 + 8.0 * D + 16.0 + 4.0 * D + 16.0 + 4.0 + 1.0) +
+                 4.0 * D * p.selcnt + (double)p.split * (8.0 * D + 4.0 + 16.0 + 8.0 + 4.0 + 1.0);
+    }
+    const double L = (double)g.host.n_leaves, N = (double)g.host.n_samples;
+    const double K = (double)g.last_n_keys;
+    if (k == "k_recon_count") return L * 16.0;
+    if (k == "k_scan_keys") return 2.0 * 8.0 * mdas::kChunkBlocks;
+    if (k == "k_recon_emit") return L * 16.0 + K * 8.0;
+    if (k == "k_rs_hist") return K * 8.0 * g.recon_sort_passes;
+    if (k == "k_rs_scan") return 8.0 * 256 * 1024 * g.recon_sort_passes;
+    if (k == "k_rs_scatter") return K * 16.0 * g.recon_sort_passes;
+    if (k == "k_tile_ranges") return K * 8.0;
+    if (k == "k_gather")
+        return K * (8.0 + 8.0 * D + 16.0) + (L > 0 ? 12.0 * N * (K / L) : 0.0) +
+               16.0 * (double)g.recon_pixels;
+    return b;
+}
+
+int mdas_kernel_stats(mdas_handle h, int index, char* name, int name_cap, int* launches,
+                      double* total_ms, double* bytes) {
+    if (!h) return fail(MDAS_ERR_INVALID, "null handle");
+    Grid& g = h->g;
+    const int n = (int)g.kstats.size();
+    if (index < 0 || index >= n) return n;
+    const auto& k = g.kstats[(size_t)index];
+    if (name && name_cap > 0) {
+        strncpy(name, k.name.c_str(), (size_t)name_cap - 1);
+        name[name_cap - 1] = 0;
+    }
+    if (launches) *launches = k.launches;
+    if (total_ms) *total_ms = k.ms;
+    if (bytes) *bytes = kernel_bytes(g, k.name);
+    return n;
+}
+
+int mdas_get_phase_times(mdas_handle h, mdas_phase_times* o) {
+    if (!h || !o) return fail(MDAS_ERR_INVALID, "null argument");
+    Grid& g = h->g;
+    memset(o, 0, sizeof(*o));
+    o->initial_ms = g.t_initial;
+    o->errors_ms = g.t_errors;
+    o->adaptive_ms = g.t_adaptive;
+    o->integrand_ms = g.t_integrand;
+    o->recon_ms = g.t_recon;
+    o->sort_ms = g.t_sort;
+    o->passes = g.passes;
+    o->final_samples = g.host.n_samples;
+    o->final_leaves = g.host.n_leaves;
+    o->bytes_initial = kernel_bytes(g, "k_initial");
+    o->bytes_errors = kernel_bytes(g, "k_errors");
+    o->bytes_adaptive = kernel_bytes(g, "k_select") + kernel_bytes(g, "k_scan_counts") +
+                        kernel_bytes(g, "k_place");
+    double br = 0.0;
+    for (const char* k : {"k_recon_count", "k_scan_keys", "k_recon_emit", "k_rs_hist", "k_rs_scan",
+                          "k_rs_scatter", "k_tile_ranges", "k_gather"})
+        br += kernel_bytes(g, k);
+    o->bytes_recon = br;
+    return MDAS_OK;
+}
+
+int mdas_set_recon_tile(mdas_handle h, int tw, int th) {
+    if (!h) return fail(MDAS_ERR_INVALID, "null handle");
+    if (tw <= 0 || th <= 0 || tw * th != 64) return fail(MDAS_ERR_INVALID, "tile must hold 64 pixels");
+    h->g.tile_w = tw;
+    h->g.tile_h = th;
+    return MDAS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,154 @@
+// Internal state of one MDAS grid (one mdas_handle) and the kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace mdas {
+
+// Device-resident counters of one grid.  Every kernel reads N / L from here,
+// so a whole sampling loop can be enqueued without host round trips.
+struct DevCounters {
+    int n_samples;       // N
+    int n_leaves;        // L
+    int n_samples_old;   // N before the last adaptive pass (new slice = [old, N))
+    int n_leaves_old;    // L before the last adaptive pass
+    unsigned gmax_bits;  // global max leaf error (float bits, errors are >= 0)
+    int new_samples;
+    int new_nodes;
+    int done;            // stopping rule reached or error: pass kernels return early
+    int status;          // 0 or an MDAS_ERR_* code raised on the device
+    int passes;          // adaptive passes performed
+    int limit;           // stopping limit min(target, max_samples)
+    int n_keys;          // reconstruction keys emitted
+    int sel_cnt;         // samples held by the leaves selected in this pass
+    int stale;           // leaves whose error was recomputed in this pass
+    int stale_vals;      // sample values read for them
+    int pad;
+};
+
+// Per-pass statistics, one row per adaptive pass (for algorithmic byte counts):
+// leaves and samples before the pass, selected and split leaves, the samples
+// held by the selected leaves, the stale leaves whose error was recomputed
+// and the sample values those read.
+struct PassStat {
+    int L, N, sel, split, selcnt, stale, stale_vals, pad;
+};
+
+// One timed kernel launch (HIP events on the launching stream).
+struct KRec {
+    const char* name;
+    hipEvent_t a, b;
+};
+
+static const int kChunkBlocks = 2048;   // workgroups of the chunked leaf kernels
+static const int kMaxPassStats = 4096;
+
+// Plain-pointer view of the device state passed to kernels by value.
+struct DevGrid {
+    float* coords;         // [cap_s * D]  AoS
+    float* values;         // [cap_s * 3]  float3 AoS
+    int4* idx;             // [cap_l]      sample slots, ~D = empty
+    float* bmn;            // [D * cap_l]  box min, SoA by dimension
+    float* bmx;            // [D * cap_l]  box max, SoA by dimension
+    float* err;            // [cap_l]      < 0 = stale
+    unsigned* seed;        // [cap_l]      LCG state, 0 = unset
+    unsigned char* flags;  // [cap_l]      bits 0-2 count, 3 selected, 4 split
+    int2* part;            // [kChunkBlocks] per-workgroup counts
+    int2* part_off;        // [kChunkBlocks] exclusive prefix of part
+    DevCounters* cnt;
+    PassStat* pstats;
+    int cap_s, cap_l;
+    int cand;
+    int b, e;
+    float alpha, sX, sY;
+};
+
+struct Grid {
+    int D = 0;
+    int cand = 4, b = 0, e = 0;
+    float alpha = 0.25f, sX = 1.0f, sY = 1.0f;
+    int cap_s = 0, cap_l = 0;
+    long long max_samples = 0;
+    bool initialized = false;
+    DevGrid dev{};
+    DevCounters host{};  // mirror after the last synchronising call
+    float gmax_host = 0.0f;
+    std::string log_path;
+    // reconstruction scratch
+    unsigned long long* keys = nullptr;
+    unsigned long long* keys_alt = nullptr;
+    long long keys_cap = 0;
+    int* tile_start = nullptr;
+    int* tile_end = nullptr;
+    long long tiles_cap = 0;
+    unsigned* rs_hist = nullptr;  // radix histogram [256 * kSortBlocks]
+    int tile_w = 8, tile_h = 8;
+    long long last_n_keys = 0;
+    int* d_scalar = nullptr;      // small device scratch
+    // timing: per-kernel events when enabled
+    bool timing = false;
+    std::vector<KRec> krecs;
+    std::vector<hipEvent_t> evpool;
+    size_t evnext = 0;
+    struct KStat {
+        std::string name;
+        int launches;
+        double ms;
+    };
+    std::vector<KStat> kstats;
+    double t_initial = 0, t_errors = 0, t_adaptive = 0, t_integrand = 0, t_recon = 0, t_sort = 0;
+    int passes = 0;
+    int passes_per_sync = 4;
+    std::vector<PassStat> pstats_host;
+    long long recon_pixels = 0;
+    int recon_sort_passes = 0;
+};
+
+// Kernel timing brackets: no-ops unless g.timing.
+inline hipEvent_t pool_event(Grid& g) {
+    if (g.evnext == g.evpool.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        g.evpool.push_back(e);
+    }
+    return g.evpool[g.evnext++];
+}
+inline void kbeg(Grid& g, hipStream_t st, const char* name) {
+    if (!g.timing) return;
+    KRec r{name, pool_event(g), pool_event(g)};
+    if (r.a) (void)hipEventRecord(r.a, st);
+    g.krecs.push_back(r);
+}
+inline void kend(Grid& g, hipStream_t st) {
+    if (!g.timing || g.krecs.empty()) return;
+    if (g.krecs.back().b) (void)hipEventRecord(g.krecs.back().b, st);
+}
+
+// Launchers (k_sampling.hip)
+hipError_t launch_initial(Grid& g, int L0, hipStream_t st);
+hipError_t launch_reset_errors(Grid& g, int standalone, hipStream_t st);
+hipError_t launch_errors(Grid& g, hipStream_t st);
+hipError_t launch_adaptive(Grid& g, hipStream_t st);
+hipError_t launch_select(Grid& g, hipStream_t st);
+hipError_t launch_scan(Grid& g, hipStream_t st);
+hipError_t launch_place(Grid& g, hipStream_t st);
+hipError_t launch_pass_begin(Grid& g, hipStream_t st);
+hipError_t launch_set_limit(Grid& g, int limit, hipStream_t st);
+hipError_t launch_fill_i32(int* p, int v, long long n, hipStream_t st);
+hipError_t launch_fill_f32(float* p, float v, long long n, hipStream_t st);
+
+// Integrand (k_integrand.hip)
+hipError_t launch_eval(Grid& g, const void* dparams, int first, int count, bool dev_range,
+                       hipStream_t st);
+
+// Reconstruction (k_recon.hip)
+hipError_t launch_integrate_gather(Grid& g, float* pixels, int W, int H, hipStream_t st,
+                                   hipEvent_t ev_sort0, hipEvent_t ev_sort1);
+hipError_t launch_integrate_scatter(Grid& g, float* pixels, int W, int H, hipStream_t st);
+hipError_t launch_bytes(const float* pixels, unsigned char* out, long long n, hipStream_t st);
+hipError_t recon_prepare(Grid& g, int W, int H, hipStream_t st);
+
+}  // namespace mdas
