@@ -166,6 +166,30 @@ int mdas_set_values(mdas_handle h, int offset, int count, const float* host_valu
  * last mdas_integrate; *n = number of keys. */
 int mdas_get_recon_keys(mdas_handle h, uint64_t* out, long long capacity, long long* n);
 
+/* ---- configuration files ----------------------------------------------------
+ * Reader of the reference's ".env" files (SDK/sutil/Environment.cpp:593-792)
+ * for the Sampler / Mdas / Film options (optixMotionBlur.cpp:107-134) plus a
+ * `Scene { kind <constant|disks|dof|mbdof|ao|ramp> seed <n> dim <d> }` block.
+ * Defaults are the reference's registered defaults. */
+typedef struct mdas_env {
+    int mdas;             /* Sampler.mdas */
+    float samples;        /* Sampler.samples (spp target) */
+    float scale_factor;   /* Mdas.scaleFactor */
+    float alpha;          /* Mdas.alpha */
+    int bits_per_dim;     /* Mdas.bitsPerDim */
+    int extra_img_bits;   /* Mdas.extraImgBits */
+    int candidates_num;   /* Mdas.candidatesNum */
+    int width, height;    /* Film.width / Film.height */
+    char filename[512];   /* Film.filename */
+    int scene_kind;       /* Scene.kind, -1 if absent */
+    uint32_t scene_seed;  /* Scene.seed */
+    int dim;              /* Scene.dim, else the scene's dimension */
+} mdas_env;
+int mdas_env_read(const char* path, mdas_env* out);
+const char* mdas_env_last_error(void);
+/* KDGrid arguments of the app (optixMotionBlur.cpp:336-350). */
+int mdas_env_config(const mdas_env* env, long long max_samples, mdas_config* cfg);
+
 const char* mdas_last_error(void);
 int mdas_device_synchronize(void);
 

@@ -62,6 +62,24 @@ class MdasPhaseTimes(ctypes.Structure):
     ]
 
 
+class MdasEnv(ctypes.Structure):
+    _fields_ = [
+        ("mdas", ctypes.c_int),
+        ("samples", ctypes.c_float),
+        ("scale_factor", ctypes.c_float),
+        ("alpha", ctypes.c_float),
+        ("bits_per_dim", ctypes.c_int),
+        ("extra_img_bits", ctypes.c_int),
+        ("candidates_num", ctypes.c_int),
+        ("width", ctypes.c_int),
+        ("height", ctypes.c_int),
+        ("filename", ctypes.c_char * 512),
+        ("scene_kind", ctypes.c_int),
+        ("scene_seed", ctypes.c_uint32),
+        ("dim", ctypes.c_int),
+    ]
+
+
 _H = ctypes.c_void_p
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -111,6 +129,9 @@ SIGNATURES = {
     "mdas_upload_state": (_I, [_H, _I, _I, _P, _P, _P, _P, _P, _P, _F]),
     "mdas_set_values": (_I, [_H, _I, _I, _P]),
     "mdas_get_recon_keys": (_I, [_H, _P, _LL, ctypes.POINTER(_LL)]),
+    "mdas_env_read": (_I, [ctypes.c_char_p, ctypes.POINTER(MdasEnv)]),
+    "mdas_env_last_error": (ctypes.c_char_p, []),
+    "mdas_env_config": (_I, [ctypes.POINTER(MdasEnv), _LL, ctypes.POINTER(MdasConfig)]),
     "mdas_last_error": (ctypes.c_char_p, []),
     "mdas_device_synchronize": (_I, []),
 }

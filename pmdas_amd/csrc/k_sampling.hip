@@ -94,6 +94,12 @@ __global__ void k_pass_begin(DevCounters* c) {
 
 __global__ void k_set_limit(DevCounters* c, int limit) { c->limit = limit; }
 
+// Re-arms the pass kernels for a standalone AdaptiveSampling call.
+__global__ void k_arm(DevCounters* c) {
+    c->done = 0;
+    c->status = 0;
+}
+
 // ------------------------------------------------------------------ errors --
 // kdgrid.cu:204-288.  Leaves with error < 0 get the relative-L1 error of their
 // samples times their volume; every leaf takes part in the global max, which
@@ -494,6 +500,11 @@ hipError_t launch_pass_begin(Grid& g, hipStream_t st) {
 
 hipError_t launch_set_limit(Grid& g, int limit, hipStream_t st) {
     hipLaunchKernelGGL(k_set_limit, dim3(1), dim3(1), 0, st, g.dev.cnt, limit);
+    return hipGetLastError();
+}
+
+hipError_t launch_arm(Grid& g, hipStream_t st) {
+    hipLaunchKernelGGL(k_arm, dim3(1), dim3(1), 0, st, g.dev.cnt);
     return hipGetLastError();
 }
 

@@ -275,6 +275,7 @@ int mdas_adaptive_sampling(mdas_handle h, void* stream) {
         HIP_TRY(hipEventCreate(&e1));
         HIP_TRY(hipEventRecord(e0, st));
     }
+    HIP_TRY(mdas::launch_arm(g, st));
     HIP_TRY(mdas::launch_adaptive(g, st));
     int rc = sync_counters(g, st);  // kdgrid.cu:753-759 reads the counters back
     if (rc) return rc;
@@ -667,7 +668,9 @@ int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream)
         int rc = sync_counters(g, st);
         if (rc) return rc;
         if (g.host.done || g.host.status) break;
-        if (enqueued > 100000) return fail(MDAS_ERR_STATE, "sampling loop did not terminate");
+        // every pass adds at least the sample of the max-error leaf, but a
+        // pathological integrand could make that one sample per pass
+        if (enqueued >= 20000) return fail(MDAS_ERR_STATE, "sampling loop exceeded 20000 passes");
     }
     g.passes = g.host.passes;
     collect_kernel_times(g);

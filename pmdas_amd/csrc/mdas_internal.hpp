@@ -137,6 +137,7 @@ hipError_t launch_scan(Grid& g, hipStream_t st);
 hipError_t launch_place(Grid& g, hipStream_t st);
 hipError_t launch_pass_begin(Grid& g, hipStream_t st);
 hipError_t launch_set_limit(Grid& g, int limit, hipStream_t st);
+hipError_t launch_arm(Grid& g, hipStream_t st);
 hipError_t launch_fill_i32(int* p, int v, long long n, hipStream_t st);
 hipError_t launch_fill_f32(float* p, float v, long long n, hipStream_t st);
 
