@@ -49,12 +49,13 @@ def run_gpu(name, steps, warmup, log=print):
                       float(w["H"]))
     sc = kdgrid.Scene(w["kind"], w["seed"], w["dim"], float(w["W"]), float(w["H"]))
     px = hip.DeviceBuffer(w["W"] * w["H"] * 16)
-    g.set_timing(True)
+    tile = None
     for i in range(warmup):
         g.Run(sc, target)
-        g.Integrate(px.ptr, w["W"], w["H"])
+        tile = g.autotune_recon(px.ptr, w["W"], w["H"])
         hip.synchronize()
-        log(f"[bench] {name}: warmup {i + 1}/{warmup} done")
+        log(f"[bench] {name}: warmup {i + 1}/{warmup} done, reconstruction tile {tile}")
+    g.set_timing(True)
     tot_mdas = tot_wall = tot_integrand = 0.0
     samples = 0
     kstats = {}
@@ -82,7 +83,7 @@ def run_gpu(name, steps, warmup, log=print):
             f"{ph['integrand_ms']:.2f} ms integrand, {(t1 - t0) * 1e3:.2f} ms wall, "
             f"{ph['passes']} passes, N={g.GetNumberOfSamples()}")
     res = dict(
-        workload=name, target=target, samples=samples // steps,
+        workload=name, target=target, samples=samples // steps, recon_tile=tile,
         leaves=g.GetNumberOfNodes(), passes=phases["passes"],
         mdas_ms=tot_mdas / steps, integrand_ms=tot_integrand / steps, wall_ms=tot_wall / steps,
         msamples_per_s=samples / (tot_mdas * 1e3),
@@ -208,6 +209,7 @@ def main():
                    "candidates_num": WORKLOADS[a.workload]["cand"],
                    "parallelism": "single GPU"},
         "mdas_ms_per_step": round(res["mdas_ms"], 3),
+        "recon_tile": res["recon_tile"],
         "integrand_ms_per_step": round(res["integrand_ms"], 3),
         "e2e_msamples_per_s": round(res["e2e_msamples_per_s"], 2),
         "samples": res["samples"], "leaves": res["leaves"], "passes": res["passes"],

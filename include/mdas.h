@@ -149,9 +149,13 @@ int mdas_get_pass_stats(mdas_handle h, int* out, int max_rows);
 int mdas_kernel_stats(mdas_handle h, int index, char* name, int name_cap, int* launches,
                       double* total_ms, double* bytes);
 
-/* Tunables of the reconstruction gather (tile of tile_w x tile_h pixels,
- * tile_w*tile_h must be 64 or 16) and of the chunked leaf kernels. */
+/* Tile of the reconstruction gather: tile_w x tile_h pixels, 16 or 64 in
+ * all (default 8 x 8).  The result does not depend on it. */
 int mdas_set_recon_tile(mdas_handle h, int tile_w, int tile_h);
+/* Reconstructs the current state into `pixels` with each candidate tile
+ * shape, keeps the fastest and returns it. */
+int mdas_autotune_recon(mdas_handle h, float* pixels, int width, int height, void* stream,
+                        int* tile_w, int* tile_h);
 
 /* State transfer for tests and tools (synchronous).
  * idx[L*4], box[L*2*dim] (per leaf: mn[dim] then mx[dim]), err[L], seed[L],

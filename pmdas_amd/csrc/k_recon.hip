@@ -1,21 +1,23 @@
 // Reconstruction: box-filter of the leaves into pixels (kdgrid.cu:483-582),
 // done as a deterministic gather instead of the reference's float atomics.
 //
-//  1. k_recon_count / k_scan_keys / k_recon_emit: every leaf emits one 64-bit
-//     key per pixel tile its pixel span touches,
+//  1. k_recon_prep: one coalesced pass over the leaves writes a 48-byte
+//     record per leaf (xy box, mean value, volume, xy area, pixel span) and
+//     counts the pixel tiles each leaf's span touches.
+//  2. k_scan_keys / k_recon_emit: every leaf emits one 64-bit key per tile,
 //         key = morton2(tile_x, tile_y) << 32 | leaf index,
 //     in leaf order (a workgroup-chunked exclusive scan gives the offsets).
-//  2. radix_sort_keys: stable LSD radix sort (8-bit digits, LDS-staged
+//  3. radix_sort_keys: stable LSD radix sort (8-bit digits, LDS-staged
 //     scatter) on the Morton bits only; since keys are emitted in leaf order
 //     the result is the total 64-bit order.
-//  3. k_tile_ranges: start / end of every tile's key segment.
-//  4. k_gather: one wave per 64-pixel tile walks the tile's leaves in index
-//     order (staged 64 at a time in LDS) and every lane accumulates its
-//     pixel.  Each contribution is computed with the reference's own
-//     expressions, including its accumulated pixel bounds (pMax += step), so a
-//     pixel is the sum of its contributions in leaf index order.
-#include "mdas_device.hpp"
-#include "mdas_internal.hpp"
+//  4. k_tile_ranges: start / end of every tile's key segment.
+//  5. k_gather: a group of 16 or 64 lanes owns a tile of 16 or 64 pixels,
+//     walks the tile's leaf records in index order (staged in LDS) and every
+//     lane accumulates its pixel.  Each contribution is computed with the
+//     reference's own expressions, including its accumulated pixel bounds
+//     (pMax += step), so a pixel is the sum of its contributions in leaf
+//     index order.
+#include "mdas_leaf.hpp"
 
 #pragma clang fp contract(off)
 
@@ -50,23 +52,61 @@ __device__ __forceinline__ int tiles_of(int x0, int x1, int y0, int y1, const Re
     return ((x1 - 1) / G.TW - x0 / G.TW + 1) * ((y1 - 1) / G.TH - y0 / G.TH + 1);
 }
 
-__global__ __launch_bounds__(256) void k_recon_count(DevGrid a, ReconGeo G) {
+__device__ __forceinline__ void unpack_span(const float4& q2, int& x0, int& x1, int& y0, int& y1) {
+    const unsigned sx = __float_as_uint(q2.y), sy = __float_as_uint(q2.z);
+    x0 = (int)(sx & 0xffffu);
+    x1 = (int)(sx >> 16);
+    y0 = (int)(sy & 0xffffu);
+    y1 = (int)(sy >> 16);
+}
+
+// Record of leaf i: q0 = (mn0, mx0, mn1, mx1), q1 = (mean value, volume),
+// q2 = ((mx0-mn0)*(mx1-mn1), x0 | x1 << 16, y0 | y1 << 16, 0).  The mean is
+// the slot-order sum times 1/n (kdgrid.cu:510-518, vec_math.h:492-496).
+template <int D>
+__global__ __launch_bounds__(256) void k_recon_prep(DevGrid a, ReconGeo G) {
     __shared__ int red[4];
     const int L = a.cnt->n_leaves;
-    const int cap = a.cap_l;
     int beg, end;
     chunk_range(L, gridDim.x, blockIdx.x, beg, end);
     int c = 0;
     for (int i = beg + threadIdx.x; i < end; i += 256) {
+        int4 id4;
+        float mn[D], mx[D];
+        load_leaf<D>(a, i, id4, mn, mx);
+        const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        int n = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (ids[q] >= 0) {
+                const float* v = a.values + (size_t)ids[q] * 3;
+                sx += v[0];
+                sy += v[1];
+                sz += v[2];
+                ++n;
+            }
+        const float inv = 1.0f / (float)n;
+        float vol = 1.0f;
+#pragma unroll
+        for (int d = 0; d < D; ++d) vol *= (mx[d] - mn[d]);
         int x0, x1, y0, y1;
-        pixel_span(a.bmn[i], a.bmx[i], a.bmn[(size_t)cap + i], a.bmx[(size_t)cap + i], G, x0, x1,
-                   y0, y1);
+        pixel_span(mn[0], mx[0], mn[1], mx[1], G, x0, x1, y0, y1);
+        // spans that start past the image are empty; clamp so they pack
+        x0 = min(x0, G.W);
+        y0 = min(y0, G.H);
         c += tiles_of(x0, x1, y0, y1, G);
+        float4* r = a.rrec + (size_t)i * 3;
+        r[0] = make_float4(mn[0], mx[0], mn[1], mx[1]);
+        r[1] = make_float4(sx * inv, sy * inv, sz * inv, vol);
+        r[2] = make_float4((mx[0] - mn[0]) * (mx[1] - mn[1]),
+                           __uint_as_float((unsigned)x0 | ((unsigned)x1 << 16)),
+                           __uint_as_float((unsigned)y0 | ((unsigned)y1 << 16)), 0.0f);
     }
     c = wave_sum(c);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
     __syncthreads();
-    if (threadIdx.x == 0) a.part[blockIdx.x] = make_int2(red[0] + red[1] + red[2] + red[3], 0);
+    if (threadIdx.x == 0) a.part[blockIdx.x] = make_int4(red[0] + red[1] + red[2] + red[3], 0, 0, 0);
 }
 
 // Exclusive scan of part[].x into part_off[].x; total -> cnt->n_keys.
@@ -105,7 +145,6 @@ __global__ __launch_bounds__(256) void k_recon_emit(DevGrid a, ReconGeo G,
                                                     unsigned long long* keys, long long cap_keys) {
     __shared__ int lds[4];
     const int L = a.cnt->n_leaves;
-    const int cap = a.cap_l;
     if ((long long)a.cnt->n_keys > cap_keys) return;  // host grows the buffer first
     int beg, end;
     chunk_range(L, gridDim.x, blockIdx.x, beg, end);
@@ -114,8 +153,7 @@ __global__ __launch_bounds__(256) void k_recon_emit(DevGrid a, ReconGeo G,
         const int i = t0 + threadIdx.x;
         int x0 = 0, x1 = 0, y0 = 0, y1 = 0, c = 0;
         if (i < end) {
-            pixel_span(a.bmn[i], a.bmx[i], a.bmn[(size_t)cap + i], a.bmx[(size_t)cap + i], G, x0,
-                       x1, y0, y1);
+            unpack_span(a.rrec[(size_t)i * 3 + 2], x0, x1, y0, y1);
             c = tiles_of(x0, x1, y0, y1, G);
         }
         int total;
@@ -127,6 +165,53 @@ __global__ __launch_bounds__(256) void k_recon_emit(DevGrid a, ReconGeo G,
         }
         base += total;
     }
+}
+
+// ---------------------------------------------------------- device scan --
+// Exclusive scan of n <= 1024*1024 unsigned ints in place: block sums, a
+// scan of the block sums, then the block-local scans with their offsets.
+__device__ __forceinline__ unsigned block_scan_u32(unsigned v, unsigned* ls, unsigned& total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        unsigned y = (unsigned)__shfl_up((int)x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) ls[w] = x;
+    __syncthreads();
+    unsigned pre = 0;
+    total = 0;
+    for (int k = 0; k < 16; ++k) {
+        if (k < w) pre += ls[k];
+        total += ls[k];
+    }
+    __syncthreads();
+    return pre + x - v;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_reduce(const unsigned* data, int n, unsigned* sums) {
+    __shared__ unsigned ls[16];
+    const int i = blockIdx.x * 1024 + threadIdx.x;
+    unsigned total;
+    (void)block_scan_u32(i < n ? data[i] : 0u, ls, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(unsigned* sums, int nb) {
+    __shared__ unsigned ls[16];
+    unsigned total;
+    const unsigned v = threadIdx.x < (unsigned)nb ? sums[threadIdx.x] : 0u;
+    const unsigned ex = block_scan_u32(v, ls, total);
+    if (threadIdx.x < (unsigned)nb) sums[threadIdx.x] = ex;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_down(unsigned* data, int n, const unsigned* sums) {
+    __shared__ unsigned ls[16];
+    const int i = blockIdx.x * 1024 + threadIdx.x;
+    unsigned total;
+    const unsigned ex = block_scan_u32(i < n ? data[i] : 0u, ls, total);
+    if (i < n) data[i] = ex + sums[blockIdx.x];
 }
 
 // ------------------------------------------------------------ radix sort --
@@ -149,33 +234,6 @@ __global__ __launch_bounds__(256) void k_rs_hist(const unsigned long long* keys,
     hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
 }
 
-__global__ __launch_bounds__(1024) void k_rs_scan(unsigned* hist, int total) {
-    __shared__ unsigned ls[16];
-    const int per = (total + 1023) / 1024;
-    const int g0 = threadIdx.x * per;
-    unsigned s = 0;
-    for (int k = 0; k < per; ++k)
-        if (g0 + k < total) s += hist[g0 + k];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    unsigned x = s;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        unsigned y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) ls[w] = x;
-    __syncthreads();
-    unsigned p = 0;
-    for (int k = 0; k < w; ++k) p += ls[k];
-    unsigned r = p + x - s;
-    for (int k = 0; k < per; ++k)
-        if (g0 + k < total) {
-            unsigned c = hist[g0 + k];
-            hist[g0 + k] = r;
-            r += c;
-        }
-}
-
 __global__ __launch_bounds__(256) void k_rs_scatter(const unsigned long long* in,
                                                     unsigned long long* out, const int* n_ptr,
                                                     int shift, const unsigned* hist) {
@@ -186,7 +244,7 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const unsigned long long* in
     __shared__ int scr[4];
     __shared__ unsigned long long stage[256];
     const int n = *n_ptr;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int w = threadIdx.x >> 6;
     base[threadIdx.x] = hist[threadIdx.x * gridDim.x + blockIdx.x];
     int beg, end;
     chunk_range(n, gridDim.x, blockIdx.x, beg, end);
@@ -215,14 +273,14 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const unsigned long long* in
             ttot[t] = tot;
         }
         __syncthreads();
-        int nvalid = min(256, end - t0);
+        const int nvalid = min(256, end - t0);
         if (valid) {
             int pre = 0;
             for (int q = 0; q < w; ++q) pre += wcnt[q][d];
             stage[toff[d] + pre + rw] = k;
         }
         __syncthreads();
-        if (threadIdx.x < nvalid) {
+        if ((int)threadIdx.x < nvalid) {
             const unsigned long long kk = stage[threadIdx.x];
             const unsigned dd = (unsigned)(kk >> shift) & 255u;
             out[base[dd] + (threadIdx.x - toff[dd])] = kk;
@@ -231,7 +289,13 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const unsigned long long* in
         base[threadIdx.x] += ttot[threadIdx.x];
         __syncthreads();
     }
-    (void)lane;
+}
+
+static void scan_u32(unsigned* data, int n, unsigned* sums, hipStream_t st) {
+    const int nb = (n + 1023) / 1024;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(1024), 0, st, data, n, sums);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, sums, nb);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(1024), 0, st, data, n, sums);
 }
 
 // Sorts keys[0, n) by bits [lo, hi); returns the buffer holding the result.
@@ -245,7 +309,7 @@ static unsigned long long* radix_sort_keys(Grid& g, int lo, int hi, hipStream_t 
         hipLaunchKernelGGL(k_rs_hist, dim3(kSortBlocks), dim3(256), 0, st, a, n_ptr, sh, g.rs_hist);
         kend(g, st);
         kbeg(g, st, "k_rs_scan");
-        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, g.rs_hist, 256 * kSortBlocks);
+        scan_u32(g.rs_hist, 256 * kSortBlocks, g.rs_hist + 256 * kSortBlocks, st);
         kend(g, st);
         kbeg(g, st, "k_rs_scatter");
         hipLaunchKernelGGL(k_rs_scatter, dim3(kSortBlocks), dim3(256), 0, st, a, b, n_ptr, sh,
@@ -270,73 +334,53 @@ __global__ __launch_bounds__(256) void k_tile_ranges(const unsigned long long* k
 }
 
 // ---------------------------------------------------------------- gather --
-template <int D>
+// GS = pixels per tile (16 or 64); a wave holds 64/GS tiles.
+template <int GS>
 __global__ __launch_bounds__(256) void k_gather(DevGrid a, ReconGeo G,
                                                 const unsigned long long* __restrict__ keys,
                                                 const int* __restrict__ tstart,
                                                 const int* __restrict__ tend, float4* pixels) {
-    __shared__ int rx0[4][64], rx1[4][64], ry0[4][64], ry1[4][64];
-    __shared__ float rmn0[4][64], rmx0[4][64], rmn1[4][64], rmx1[4][64];
-    __shared__ float rvol[4][64], rdxy[4][64], rax[4][64], ray[4][64], raz[4][64];
+    __shared__ float4 r0[4][64], r1[4][64], r2[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int tile = blockIdx.x * 4 + w;
-    if (tile >= G.tilesX * G.tilesY) return;
-    const int tx = tile % G.tilesX, ty = tile / G.tilesX;
-    const int px = tx * G.TW + lane % G.TW;
-    const int py = ty * G.TH + lane / G.TW;
-    const unsigned code = morton2(tx, ty);
-    const int s0 = tstart[code], s1 = tend[code];
-    const int cap = a.cap_l;
+    const int grp = lane / GS, gl = lane % GS;      // tile within the wave, lane within the tile
+    const int tile = (blockIdx.x * 4 + w) * (64 / GS) + grp;
+    const int ntiles = G.tilesX * G.tilesY;
+    const bool live = tile < ntiles;
+    const int tx = live ? tile % G.tilesX : 0, ty = live ? tile / G.tilesX : 0;
+    const int px = tx * G.TW + gl % G.TW;
+    const int py = ty * G.TH + gl / G.TW;
+    int s0 = 0, s1 = 0;
+    if (live) {
+        const unsigned code = morton2(tx, ty);
+        s0 = tstart[code];
+        s1 = tend[code];
+    }
+    // rounds of GS records until the longest tile of the wave is done
+    int rounds = (s1 - s0 + GS - 1) / GS;
+#pragma unroll
+    for (int o = GS; o < 64; o <<= 1) rounds = max(rounds, __shfl_xor(rounds, o, 64));
     float accx = 0.0f, accy = 0.0f, accz = 0.0f;
-    for (int base = s0; base < s1; base += 64) {
-        const int nrec = min(64, s1 - base);
-        if (lane < nrec) {
-            const int i = (int)(unsigned)(keys[base + lane] & 0xffffffffull);
-            float mn[D], mx[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                mn[d] = a.bmn[(size_t)d * cap + i];
-                mx[d] = a.bmx[(size_t)d * cap + i];
-            }
-            int x0, x1, y0, y1;
-            pixel_span(mn[0], mx[0], mn[1], mx[1], G, x0, x1, y0, y1);
-            const int4 id4 = a.idx[i];
-            const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
-            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-            int n = 0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (ids[q] >= 0) {
-                    const float* v = a.values + (size_t)ids[q] * 3;
-                    sx += v[0];
-                    sy += v[1];
-                    sz += v[2];
-                    ++n;
-                }
-            const float inv = 1.0f / (float)n;
-            float vol = 1.0f;
-#pragma unroll
-            for (int d = 0; d < D; ++d) vol *= (mx[d] - mn[d]);
-            rx0[w][lane] = x0;
-            rx1[w][lane] = x1;
-            ry0[w][lane] = y0;
-            ry1[w][lane] = y1;
-            rmn0[w][lane] = mn[0];
-            rmx0[w][lane] = mx[0];
-            rmn1[w][lane] = mn[1];
-            rmx1[w][lane] = mx[1];
-            rvol[w][lane] = vol;
-            rdxy[w][lane] = (mx[0] - mn[0]) * (mx[1] - mn[1]);
-            rax[w][lane] = sx * inv;
-            ray[w][lane] = sy * inv;
-            raz[w][lane] = sz * inv;
+    const int gbase = grp * GS;
+    for (int rd = 0; rd < rounds; ++rd) {
+        const int base = s0 + rd * GS;
+        const int nrec = max(0, min(GS, s1 - base));
+        if (gl < nrec) {
+            const int i = (int)(unsigned)(keys[base + gl] & 0xffffffffull);
+            const float4* r = a.rrec + (size_t)i * 3;
+            r0[w][gbase + gl] = r[0];
+            r1[w][gbase + gl] = r[1];
+            r2[w][gbase + gl] = r[2];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         for (int j = 0; j < nrec; ++j) {
-            const int x0 = rx0[w][j], x1 = rx1[w][j], y0 = ry0[w][j], y1 = ry1[w][j];
+            int x0, x1, y0, y1;
+            const float4 q2 = r2[w][gbase + j];
+            unpack_span(q2, x0, x1, y0, y1);
             if (px >= x0 && px < x1 && py >= y0 && py < y1) {
+                const float4 q0 = r0[w][gbase + j];
+                const float4 q1 = r1[w][gbase + j];
                 // accumulated pixel bounds exactly as kdgrid.cu:536-576 walks them
                 float pMinX = (float)x0 * G.sX / G.fW;
                 float pMaxX = pMinX + G.stepX;
@@ -350,15 +394,15 @@ __global__ __launch_bounds__(256) void k_gather(DevGrid a, ReconGeo G,
                     pMinY = pMaxY;
                     pMaxY += G.stepY;
                 }
-                const float clox = fmaxf(pMinX, rmn0[w][j]);
-                const float chix = fminf(pMaxX, rmx0[w][j]);
-                const float cloy = fmaxf(pMinY, rmn1[w][j]);
-                const float chiy = fminf(pMaxY, rmx1[w][j]);
+                const float clox = fmaxf(pMinX, q0.x);
+                const float chix = fminf(pMaxX, q0.y);
+                const float cloy = fmaxf(pMinY, q0.z);
+                const float chiy = fminf(pMaxY, q0.w);
                 if (clox < chix && cloy < chiy) {
-                    const float v = (chix - clox) * (chiy - cloy) * rvol[w][j] / rdxy[w][j];
-                    accx += (rax[w][j] * v) * G.invPixArea;
-                    accy += (ray[w][j] * v) * G.invPixArea;
-                    accz += (raz[w][j] * v) * G.invPixArea;
+                    const float v = (chix - clox) * (chiy - cloy) * q1.w / q2.x;
+                    accx += (q1.x * v) * G.invPixArea;
+                    accy += (q1.y * v) * G.invPixArea;
+                    accz += (q1.z * v) * G.invPixArea;
                 }
             }
         }
@@ -366,7 +410,7 @@ __global__ __launch_bounds__(256) void k_gather(DevGrid a, ReconGeo G,
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    if (px < G.W && py < G.H) pixels[(size_t)py * G.W + px] = make_float4(accx, accy, accz, 0.0f);
+    if (live && px < G.W && py < G.H) pixels[(size_t)py * G.W + px] = make_float4(accx, accy, accz, 0.0f);
 }
 
 // -------------------------------------------------------- scatter (atomics) --
@@ -374,15 +418,10 @@ __global__ __launch_bounds__(256) void k_gather(DevGrid a, ReconGeo G,
 template <int D>
 __global__ __launch_bounds__(256) void k_scatter(DevGrid a, ReconGeo G, float4* pixels) {
     const int L = a.cnt->n_leaves;
-    const int cap = a.cap_l;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < L; i += gridDim.x * 256) {
+        int4 id4;
         float mn[D], mx[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            mn[d] = a.bmn[(size_t)d * cap + i];
-            mx[d] = a.bmx[(size_t)d * cap + i];
-        }
-        const int4 id4 = a.idx[i];
+        load_leaf<D>(a, i, id4, mn, mx);
         const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         int n = 0;
@@ -488,13 +527,18 @@ static ReconGeo make_geo(const Grid& g, int W, int H) {
     default: return hipErrorInvalidValue;              \
     }
 
-// Count + scan of the reconstruction keys, then (synchronising) make sure the
-// key buffers hold them.
+// Per-leaf records, tile counts and their scan, then (synchronising) make
+// sure the key buffers hold the keys.
 hipError_t recon_prepare(Grid& g, int W, int H, hipStream_t st) {
     ReconGeo G = make_geo(g, W, H);
     g.recon_pixels = (long long)W * H;
-    kbeg(g, st, "k_recon_count");
-    hipLaunchKernelGGL(k_recon_count, dim3(kChunkBlocks), dim3(256), 0, st, g.dev, G);
+    if (!g.dev.rrec) {
+        hipError_t e0 = hipMalloc(&g.dev.rrec, sizeof(float4) * 3 * (size_t)g.cap_l);
+        if (e0 != hipSuccess) return e0;
+    }
+    kbeg(g, st, "k_recon_prep");
+    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_recon_prep<DD>, dim3(kChunkBlocks), dim3(256), 0, st,
+                                            g.dev, G));
     kend(g, st);
     kbeg(g, st, "k_scan_keys");
     hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, st, g.dev, kChunkBlocks);
@@ -506,8 +550,8 @@ hipError_t recon_prepare(Grid& g, int W, int H, hipStream_t st) {
     if (e != hipSuccess) return e;
     g.last_n_keys = nk;
     if (nk > g.keys_cap) {
-        if (g.keys) hipFree(g.keys);
-        if (g.keys_alt) hipFree(g.keys_alt);
+        if (g.keys) (void)hipFree(g.keys);
+        if (g.keys_alt) (void)hipFree(g.keys_alt);
         g.keys = g.keys_alt = nullptr;
         long long cap = (long long)nk + nk / 4 + 1024;
         e = hipMalloc(&g.keys, sizeof(unsigned long long) * cap);
@@ -518,8 +562,8 @@ hipError_t recon_prepare(Grid& g, int W, int H, hipStream_t st) {
     }
     long long tiles = 1ll << (2 * G.tbits);
     if (tiles > g.tiles_cap) {
-        if (g.tile_start) hipFree(g.tile_start);
-        if (g.tile_end) hipFree(g.tile_end);
+        if (g.tile_start) (void)hipFree(g.tile_start);
+        if (g.tile_end) (void)hipFree(g.tile_end);
         e = hipMalloc(&g.tile_start, sizeof(int) * tiles);
         if (e != hipSuccess) return e;
         e = hipMalloc(&g.tile_end, sizeof(int) * tiles);
@@ -527,7 +571,8 @@ hipError_t recon_prepare(Grid& g, int W, int H, hipStream_t st) {
         g.tiles_cap = tiles;
     }
     if (!g.rs_hist) {
-        e = hipMalloc(&g.rs_hist, sizeof(unsigned) * 256 * kSortBlocks);
+        // histograms plus the block sums of their scan
+        e = hipMalloc(&g.rs_hist, sizeof(unsigned) * (256 * kSortBlocks + 1024));
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -559,10 +604,17 @@ hipError_t launch_integrate_gather(Grid& g, float* pixels, int W, int H, hipStre
                        g.tile_start, g.tile_end);
     kend(g, st);
     const int ntiles = G.tilesX * G.tilesY;
+    const int gs = G.TW * G.TH;
+    const int tiles_per_block = 4 * (64 / gs);
     kbeg(g, st, "k_gather");
-    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_gather<DD>, dim3((ntiles + 3) / 4), dim3(256), 0, st,
-                                            g.dev, G, g.keys, g.tile_start, g.tile_end,
-                                            (float4*)pixels));
+    if (gs == 64)
+        hipLaunchKernelGGL(k_gather<64>, dim3((ntiles + tiles_per_block - 1) / tiles_per_block),
+                           dim3(256), 0, st, g.dev, G, g.keys, g.tile_start, g.tile_end,
+                           (float4*)pixels);
+    else
+        hipLaunchKernelGGL(k_gather<16>, dim3((ntiles + tiles_per_block - 1) / tiles_per_block),
+                           dim3(256), 0, st, g.dev, G, g.keys, g.tile_start, g.tile_end,
+                           (float4*)pixels);
     kend(g, st);
     return hipGetLastError();
 }

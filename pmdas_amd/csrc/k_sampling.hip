@@ -3,11 +3,12 @@
 // median split) with indices allocated by a deterministic device-wide scan.
 //
 // Layout (HBM): sample coordinates AoS float[D] and values float3 (the
-// integrand contract); leaf slots int4; leaf boxes SoA by dimension so that
-// a wave reads one dimension of 64 consecutive leaves as 256 contiguous bytes;
-// errors, LCG states and a one-byte flag word per leaf.
-#include "mdas_device.hpp"
-#include "mdas_internal.hpp"
+// integrand contract); one 48/64-byte record per leaf (int4 slots + box) so a
+// leaf touched at a random index costs one cache-line segment; errors, LCG
+// states and a one-byte flag word per leaf as separate streams for the
+// per-pass sweep over all leaves.  Cross-workgroup reductions go through
+// per-workgroup partials (no same-address atomics).
+#include "mdas_leaf.hpp"
 
 #pragma clang fp contract(off)
 
@@ -23,7 +24,6 @@ namespace mdas {
 template <int D>
 __global__ __launch_bounds__(256) void k_initial(DevGrid a, int L0) {
     __shared__ float stage[256 * 4 * D];
-    const int cap = a.cap_l;
     for (int base = blockIdx.x * 256; base < L0; base += gridDim.x * 256) {
         const int i = base + threadIdx.x;
         if (i < L0) {
@@ -38,15 +38,13 @@ __global__ __launch_bounds__(256) void k_initial(DevGrid a, int L0) {
 #pragma unroll
                 for (int d = 0; d < D; ++d) stage[threadIdx.x * 4 * D + j * D + d] = off[d] + r[d] * ext[d];
             }
-            a.idx[i] = make_int4(4 * i, 4 * i + 1, 4 * i + 2, 4 * i + 3);
+            float mx[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) mx[d] = off[d] + ext[d];
+            store_leaf<D>(a, i, make_int4(4 * i, 4 * i + 1, 4 * i + 2, 4 * i + 3), off, mx);
             a.seed[i] = s;
             a.err[i] = -1.0f;
             a.flags[i] = 4;
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                a.bmn[(size_t)d * cap + i] = off[d];
-                a.bmx[(size_t)d * cap + i] = off[d] + ext[d];
-            }
         }
         __syncthreads();
         const int nl = min(256, L0 - base);
@@ -72,6 +70,7 @@ __global__ void k_initial_counters(DevCounters* c, int L0) {
     c->sel_cnt = 0;
     c->stale = 0;
     c->stale_vals = 0;
+    c->carry_bits = 0;
 }
 
 // ComputeErrors prologue (kdgrid.cu:684-685): g_error = 0.  A standalone call
@@ -82,14 +81,15 @@ __global__ void k_reset_errors(DevCounters* c, int standalone) {
 }
 
 // One pass of the device-side loop (optixMotionBlur.cpp:781): stop when
-// N >= min(target, max_samples).
+// N >= min(target, max_samples).  gmax starts at the max error of the leaves
+// the previous pass did not split (their errors are not re-estimated).
 __global__ void k_pass_begin(DevCounters* c) {
     if (c->done) return;
     if (c->n_samples >= c->limit) {
         c->done = 1;
         return;
     }
-    c->gmax_bits = 0;
+    c->gmax_bits = c->passes == 0 ? 0u : c->carry_bits;
 }
 
 __global__ void k_set_limit(DevCounters* c, int limit) { c->limit = limit; }
@@ -101,69 +101,60 @@ __global__ void k_arm(DevCounters* c) {
 }
 
 // ------------------------------------------------------------------ errors --
-// kdgrid.cu:204-288.  Leaves with error < 0 get the relative-L1 error of their
-// samples times their volume; every leaf takes part in the global max, which
-// is reduced per wave, per workgroup and then with one atomicMax on the float
-// bits (order-independent, so exact).
+// kdgrid.cu:237-266: relative-L1 error of a leaf's samples times its volume.
+// *nvals returns the number of sample values read.
 template <int D>
-__global__ __launch_bounds__(256) void k_errors(DevGrid a) {
-    __shared__ float wmax[4];
-    __shared__ int wst[8];
-    if (a.cnt->done) return;
-    const int L = a.cnt->n_leaves;
-    const int cap = a.cap_l;
-    float m = 0.0f;
-    int n_stale = 0, n_vals = 0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < L; i += gridDim.x * 256) {
-        float e = a.err[i];
-        if (e < 0.0f) {
-            const int4 id = a.idx[i];
-            const int ids[4] = {id.x, id.y, id.z, id.w};
-            float vx[4] = {0.0f, 0.0f, 0.0f, 0.0f}, vy[4] = {0.0f, 0.0f, 0.0f, 0.0f},
-                  vz[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-            int n = 0;
+__device__ __forceinline__ float leaf_error(const DevGrid& a, int i, int* nvals) {
+    int4 id;
+    float mn[D], mx[D];
+    load_leaf<D>(a, i, id, mn, mx);
+    const int ids[4] = {id.x, id.y, id.z, id.w};
+    float vx[4] = {0.0f, 0.0f, 0.0f, 0.0f}, vy[4] = {0.0f, 0.0f, 0.0f, 0.0f},
+          vz[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    int n = 0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (ids[q] >= 0) {
-                    const float* v = a.values + (size_t)ids[q] * 3;
-                    vx[q] = v[0];
-                    vy[q] = v[1];
-                    vz[q] = v[2];
-                    sx += vx[q];
-                    sy += vy[q];
-                    sz += vz[q];
-                    ++n;
-                }
-            }
-            // float3 /= float is a reciprocal multiply (sutil/vec_math.h:492-496)
-            const float inv = 1.0f / (float)n;
-            const float ax = sx * inv, ay = sy * inv, az = sz * inv;
-            float dx = 0.0f, dy = 0.0f, dz = 0.0f;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (q < n) {
-                    dx += fabsf(vx[q] - ax);
-                    dy += fabsf(vy[q] - ay);
-                    dz += fabsf(vz[q] - az);
-                }
-            }
-            float vol = 1.0f;
-#pragma unroll
-            for (int d = 0; d < D; ++d) vol *= (a.bmx[(size_t)d * cap + i] - a.bmn[(size_t)d * cap + i]);
-            e = 0.0f;
-            if (ax != 0.0f) e += dx / ax;
-            if (ay != 0.0f) e += dy / ay;
-            if (az != 0.0f) e += dz / az;
-            e /= (float)n;
-            e += 1.0e-5f;
-            e *= vol;
-            a.err[i] = e;
-            n_stale += 1;
-            n_vals += n;
+    for (int q = 0; q < 4; ++q) {
+        if (ids[q] >= 0) {
+            const float* v = a.values + (size_t)ids[q] * 3;
+            vx[q] = v[0];
+            vy[q] = v[1];
+            vz[q] = v[2];
+            sx += vx[q];
+            sy += vy[q];
+            sz += vz[q];
+            ++n;
         }
-        m = fmaxf(m, e);
     }
+    // float3 /= float is a reciprocal multiply (sutil/vec_math.h:492-496)
+    const float inv = 1.0f / (float)n;
+    const float ax = sx * inv, ay = sy * inv, az = sz * inv;
+    float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (q < n) {
+            dx += fabsf(vx[q] - ax);
+            dy += fabsf(vy[q] - ay);
+            dz += fabsf(vz[q] - az);
+        }
+    }
+    float vol = 1.0f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) vol *= (mx[d] - mn[d]);
+    float e = 0.0f;
+    if (ax != 0.0f) e += dx / ax;
+    if (ay != 0.0f) e += dy / ay;
+    if (az != 0.0f) e += dz / az;
+    e /= (float)n;
+    e += 1.0e-5f;
+    e *= vol;
+    *nvals = n;
+    return e;
+}
+
+// Workgroup reduction of (max, stale count, values read) into epart[block].
+__device__ __forceinline__ void errors_epilogue(const DevGrid& a, float m, int n_stale, int n_vals,
+                                                float* wmax, int* wst) {
     m = wave_max(m);
     n_stale = wave_sum(n_stale);
     n_vals = wave_sum(n_vals);
@@ -174,24 +165,114 @@ __global__ __launch_bounds__(256) void k_errors(DevGrid a) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        float b = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
-        atomicMax(&a.cnt->gmax_bits, __float_as_uint(b));
-        const int ts = wst[0] + wst[1] + wst[2] + wst[3];
-        const int tv = wst[4] + wst[5] + wst[6] + wst[7];
-        if (ts) {
-            atomicAdd(&a.cnt->stale, ts);
-            atomicAdd(&a.cnt->stale_vals, tv);
+        const float b = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+        a.epart[blockIdx.x] = make_uint4(__float_as_uint(b), (unsigned)(wst[0] + wst[1] + wst[2] + wst[3]),
+                                         (unsigned)(wst[4] + wst[5] + wst[6] + wst[7]), 0u);
+    }
+}
+
+// kdgrid.cu:204-288 as the reference runs it: every leaf with error < 0 is
+// re-estimated and every leaf takes part in the global max.  The max of
+// non-negative floats is the max of their bit patterns, so the reduction is
+// exact in any order.
+template <int D>
+__global__ __launch_bounds__(256) void k_errors(DevGrid a) {
+    __shared__ float wmax[4];
+    __shared__ int wst[8];
+    if (a.cnt->done) return;
+    const int L = a.cnt->n_leaves;
+    float m = 0.0f;
+    int n_stale = 0, n_vals = 0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < L; i += gridDim.x * 256) {
+        float e = a.err[i];
+        if (e < 0.0f) {
+            int n;
+            e = leaf_error<D>(a, i, &n);
+            a.err[i] = e;
+            n_stale += 1;
+            n_vals += n;
         }
+        m = fmaxf(m, e);
+    }
+    errors_epilogue(a, m, n_stale, n_vals, wmax, wst);
+}
+
+// The same result inside the sampling loop without the sweep over all
+// leaves: the only leaves with error < 0 are the two children of every leaf
+// the last pass split (split_list[t] and L_old + t); the max over the other
+// leaves was carried by the selection kernel into gmax by k_pass_begin.  The
+// first pass after the initial grid re-estimates every leaf.
+template <int D>
+__global__ __launch_bounds__(256) void k_errors_stale(DevGrid a) {
+    __shared__ float wmax[4];
+    __shared__ int wst[8];
+    if (a.cnt->done) return;
+    const bool first = a.cnt->passes == 0;
+    const int nn = first ? a.cnt->n_leaves : a.cnt->new_nodes;
+    const int n_items = first ? nn : 2 * nn;
+    const int L_old = a.cnt->n_leaves_old;
+    float m = 0.0f;
+    int n_stale = 0, n_vals = 0;
+    for (int t = blockIdx.x * 256 + threadIdx.x; t < n_items; t += gridDim.x * 256) {
+        const int i = first ? t : (t < nn ? a.split_list[t] : L_old + (t - nn));
+        int n;
+        const float e = leaf_error<D>(a, i, &n);
+        a.err[i] = e;
+        n_stale += 1;
+        n_vals += n;
+        m = fmaxf(m, e);
+    }
+    errors_epilogue(a, m, n_stale, n_vals, wmax, wst);
+}
+
+// Folds the error partials into gmax and the per-pass statistics.
+__global__ __launch_bounds__(1024) void k_err_reduce(DevGrid a, int G) {
+    __shared__ unsigned sm[16];
+    __shared__ int ss[16], sv[16];
+    if (a.cnt->done) return;
+    unsigned m = 0;
+    int st = 0, va = 0;
+    for (int k = threadIdx.x; k < G; k += 1024) {
+        const uint4 p = a.epart[k];
+        m = max(m, p.x);
+        st += (int)p.y;
+        va += (int)p.z;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        m = max(m, (unsigned)__shfl_xor((int)m, o, 64));
+        st += __shfl_xor(st, o, 64);
+        va += __shfl_xor(va, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sm[threadIdx.x >> 6] = m;
+        ss[threadIdx.x >> 6] = st;
+        sv[threadIdx.x >> 6] = va;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned mm = a.cnt->gmax_bits;
+        int s1 = 0, s2 = 0;
+        for (int k = 0; k < 16; ++k) {
+            mm = max(mm, sm[k]);
+            s1 += ss[k];
+            s2 += sv[k];
+        }
+        a.cnt->gmax_bits = mm;
+        a.cnt->stale += s1;
+        a.cnt->stale_vals += s2;
     }
 }
 
 // --------------------------------------------------------------- selection --
 // kdgrid.cu:316-326.  Per leaf: the seed (tea4(i,0) when unset), one draw u,
 // selected iff pow(err / g_error, alpha) >= u.  A selected full leaf splits.
-// Writes the advanced LCG state and the flags, and one (selected, split)
-// count per workgroup for the allocation scan.
+// Writes the advanced LCG state and the flags, and per workgroup the
+// (selected, split) counts for the allocation scan, the samples held by the
+// selected leaves, and the max error of the leaves that do not split.
 __global__ __launch_bounds__(256) void k_select(DevGrid a) {
     __shared__ int red[12];
+    __shared__ float rmax[4];
     if (a.cnt->done) return;
     const int L = a.cnt->n_leaves;
     const float gmax = __uint_as_float(a.cnt->gmax_bits);
@@ -199,6 +280,7 @@ __global__ __launch_bounds__(256) void k_select(DevGrid a) {
     int beg, end;
     chunk_range(L, gridDim.x, blockIdx.x, beg, end);
     int ns = 0, nl = 0, nc = 0;
+    float carry = 0.0f;
     for (int i = beg + threadIdx.x; i < end; i += 256) {
         const float e = a.err[i];
         unsigned s = a.seed[i];
@@ -212,53 +294,67 @@ __global__ __launch_bounds__(256) void k_select(DevGrid a) {
         ns += sel ? 1 : 0;
         nl += split ? 1 : 0;
         nc += sel ? (int)cnt : 0;
+        if (!split) carry = fmaxf(carry, e);
     }
     ns = wave_sum(ns);
     nl = wave_sum(nl);
     nc = wave_sum(nc);
+    carry = wave_max(carry);
     if ((threadIdx.x & 63) == 0) {
         red[threadIdx.x >> 6] = ns;
         red[4 + (threadIdx.x >> 6)] = nl;
         red[8 + (threadIdx.x >> 6)] = nc;
+        rmax[threadIdx.x >> 6] = carry;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        a.part[blockIdx.x] = make_int2(red[0] + red[1] + red[2] + red[3], red[4] + red[5] + red[6] + red[7]);
-        const int tc = red[8] + red[9] + red[10] + red[11];
-        if (tc) atomicAdd(&a.cnt->sel_cnt, tc);
+        const float cm = fmaxf(fmaxf(rmax[0], rmax[1]), fmaxf(rmax[2], rmax[3]));
+        a.part[blockIdx.x] = make_int4(red[0] + red[1] + red[2] + red[3], red[4] + red[5] + red[6] + red[7],
+                                       red[8] + red[9] + red[10] + red[11], (int)__float_as_uint(cm));
     }
 }
 
 // Exclusive scan of the per-workgroup counts (one workgroup of 1024 threads),
 // then the capacity check and the counter update (kdgrid.cu:753-759).
 __global__ __launch_bounds__(1024) void k_scan_counts(DevGrid a, int G) {
-    __shared__ int ls[16], ll[16];
+    __shared__ int ls[16], ll[16], lc[16];
+    __shared__ unsigned lm[16];
     if (a.cnt->done) return;
     const int per = (G + 1023) / 1024;
     const int g0 = threadIdx.x * per;
-    int s = 0, l = 0;
+    int s = 0, l = 0, c = 0;
+    unsigned mx = 0;
     for (int k = 0; k < per; ++k)
         if (g0 + k < G) {
-            int2 p = a.part[g0 + k];
+            int4 p = a.part[g0 + k];
             s += p.x;
             l += p.y;
+            c += p.z;
+            mx = max(mx, (unsigned)p.w);
         }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int xs = s, xl = l;
+    int xs = s, xl = l, xc = c;
+    unsigned xm = mx;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        int ys = __shfl_up(xs, o, 64), yl = __shfl_up(xl, o, 64);
+        int ys = __shfl_up(xs, o, 64), yl = __shfl_up(xl, o, 64), yc = __shfl_up(xc, o, 64);
+        unsigned ym = (unsigned)__shfl_up((int)xm, o, 64);
         if (lane >= o) {
             xs += ys;
             xl += yl;
+            xc += yc;
+            xm = max(xm, ym);
         }
     }
     if (lane == 63) {
         ls[w] = xs;
         ll[w] = xl;
+        lc[w] = xc;
+        lm[w] = xm;
     }
     __syncthreads();
-    int ps = 0, pl = 0, ts = 0, tl = 0;
+    int ps = 0, pl = 0, ts = 0, tl = 0, tc = 0;
+    unsigned tm = 0;
     for (int k = 0; k < 16; ++k) {
         if (k < w) {
             ps += ls[k];
@@ -266,35 +362,38 @@ __global__ __launch_bounds__(1024) void k_scan_counts(DevGrid a, int G) {
         }
         ts += ls[k];
         tl += ll[k];
+        tc += lc[k];
+        tm = max(tm, lm[k]);
     }
     int rs = ps + xs - s, rl = pl + xl - l;
     for (int k = 0; k < per; ++k)
         if (g0 + k < G) {
-            int2 p = a.part[g0 + k];
+            int4 p = a.part[g0 + k];
             a.part_off[g0 + k] = make_int2(rs, rl);
             rs += p.x;
             rl += p.y;
         }
     if (threadIdx.x == 0) {
-        DevCounters* c = a.cnt;
-        const int N = c->n_samples, L = c->n_leaves;
+        DevCounters* cn = a.cnt;
+        const int N = cn->n_samples, L = cn->n_leaves;
         if ((long long)N + ts > a.cap_s || (long long)L + tl > a.cap_l) {
-            c->status = MDAS_ERR_CAPACITY_DEV;
-            c->done = 1;
+            cn->status = MDAS_ERR_CAPACITY_DEV;
+            cn->done = 1;
         } else {
-            c->n_samples_old = N;
-            c->n_leaves_old = L;
-            c->n_samples = N + ts;
-            c->n_leaves = L + tl;
-            c->new_samples = ts;
-            c->new_nodes = tl;
-            const int p = c->passes;
-            if (p < kMaxPassStats) a.pstats[p] = PassStat{L, N, ts, tl, c->sel_cnt, c->stale, c->stale_vals, 0};
-            c->passes = p + 1;
+            cn->n_samples_old = N;
+            cn->n_leaves_old = L;
+            cn->n_samples = N + ts;
+            cn->n_leaves = L + tl;
+            cn->new_samples = ts;
+            cn->new_nodes = tl;
+            cn->carry_bits = tm;
+            const int p = cn->passes;
+            if (p < kMaxPassStats) a.pstats[p] = PassStat{L, N, ts, tl, tc, cn->stale, cn->stale_vals, 0};
+            cn->passes = p + 1;
         }
-        c->sel_cnt = 0;
-        c->stale = 0;
-        c->stale_vals = 0;
+        cn->sel_cnt = 0;
+        cn->stale = 0;
+        cn->stale_vals = 0;
     }
 }
 
@@ -306,17 +405,15 @@ __global__ __launch_bounds__(1024) void k_scan_counts(DevGrid a, int G) {
 // the largest box axis, keeping the left half in place and the right half
 // at leaf index m.
 template <int D>
-__device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m, unsigned cnt) {
-    const int cap = a.cap_l;
+__device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m, unsigned cnt,
+                                           int L) {
+    constexpr int Q = leaf_quads(D);
     unsigned s = a.seed[i];
+    int4 id4;
     float mn[D], mx[D], diag[D];
+    load_leaf<D>(a, i, id4, mn, mx);
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-        mn[d] = a.bmn[(size_t)d * cap + i];
-        mx[d] = a.bmx[(size_t)d * cap + i];
-        diag[d] = mx[d] - mn[d];
-    }
-    const int4 id4 = a.idx[i];
+    for (int d = 0; d < D; ++d) diag[d] = mx[d] - mn[d];
     const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
     float p[4][D];
 #pragma unroll
@@ -340,7 +437,9 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
         for (int d = 0; d < D; ++d) c[d] = rnd(s);
 #pragma unroll
         for (int d = 0; d < D; ++d) c[d] = mn[d] + c[d] * diag[d];
-        float minDist = 3.402823466e+38f;
+        // min over samples of sqrt(d2) = sqrt(min over samples of d2), since
+        // sqrt is monotone and correctly rounded; one sqrt per candidate.
+        float minD2 = __int_as_float(0x7f800000);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             if (ids[q] >= 0) {
@@ -350,10 +449,11 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
                     float t = c[d] - p[q][d];
                     acc += t * t;
                 }
-                const float dist = __builtin_sqrtf(acc);
-                if (minDist > dist) minDist = dist;
+                if (minD2 > acc) minD2 = acc;
             }
         }
+        const float minDist = minD2 < __int_as_float(0x7f800000) ? __builtin_sqrtf(minD2)
+                                                                 : 3.402823466e+38f;
         if (maxDist < minDist) {
             maxDist = minDist;
 #pragma unroll
@@ -371,7 +471,7 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
         else if (cnt == 1u) n4.y = k;
         else if (cnt == 2u) n4.z = k;
         else n4.w = k;
-        a.idx[i] = n4;
+        store_slots(a, i, Q, n4);
         a.flags[i] = (unsigned char)(cnt + 1u);
     } else {
         // kdgrid.cu:399-471: split.  LargestAxis scans from the last axis down
@@ -418,41 +518,62 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
         }
         const float pos = 0.5f * (key[1] + key[2]);
         const int E = ~D;
-        a.idx[i] = make_int4(v[0], v[1], E, E);
-        a.idx[m] = make_int4(v[2], v[3], v[4], E);
+        float cmn[D], cmx[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            a.bmn[(size_t)d * cap + m] = d == axis ? pos : mn[d];
-            a.bmx[(size_t)d * cap + m] = mx[d];
+            cmn[d] = d == axis ? pos : mn[d];
+            cmx[d] = d == axis ? pos : mx[d];
         }
-        a.bmx[(size_t)axis * cap + i] = pos;
+        store_leaf<D>(a, i, make_int4(v[0], v[1], E, E), mn, cmx);
+        store_leaf<D>(a, m, make_int4(v[2], v[3], v[4], E), cmn, mx);
         a.err[i] = -1.0f;
         a.err[m] = -1.0f;
         a.seed[m] = 0u;
         a.flags[i] = 2;
         a.flags[m] = 3;
+        a.split_list[m - L] = i;  // the split leaves in leaf order (right child m)
     }
     a.seed[i] = s;
 }
 
+// Selected leaves of a workgroup's chunk are ranked in leaf order and queued
+// in LDS; whenever the queue holds more than three waves' worth they are
+// placed with all lanes busy (the selection rate is typically 20-60 %).
 template <int D>
 __global__ __launch_bounds__(256) void k_place(DevGrid a) {
     __shared__ int lds[8];
+    __shared__ int q_leaf[1024], q_k[1024], q_m[1024];
+    __shared__ unsigned char q_cnt[1024];
     if (a.cnt->done) return;
     const int L = a.cnt->n_leaves_old;
     const int N0 = a.cnt->n_samples_old;
     int beg, end;
     chunk_range(L, gridDim.x, blockIdx.x, beg, end);
     int2 base = a.part_off[blockIdx.x];
+    int qn = 0;
     for (int t0 = beg; t0 < end; t0 += 256) {
         const int i = t0 + threadIdx.x;
         const unsigned f = i < end ? a.flags[i] : 0u;
         const bool sel = (f & 8u) != 0u, split = (f & 16u) != 0u;
         int rs, rl, ts, tl;
         block_rank2(sel, split, lds, rs, rl, ts, tl);
-        if (sel) place_leaf<D>(a, i, N0 + base.x + rs, split ? L + base.y + rl : -1, f & 7u);
+        if (sel) {
+            const int slot = qn + rs;
+            q_leaf[slot] = i;
+            q_k[slot] = N0 + base.x + rs;
+            q_m[slot] = split ? L + base.y + rl : -1;
+            q_cnt[slot] = (unsigned char)(f & 7u);
+        }
         base.x += ts;
         base.y += tl;
+        qn += ts;
+        if (qn > 768 || t0 + 256 >= end) {
+            __syncthreads();
+            for (int j = threadIdx.x; j < qn; j += 256)
+                place_leaf<D>(a, q_leaf[j], q_k[j], q_m[j], q_cnt[j], L);
+            qn = 0;
+            __syncthreads();
+        }
     }
 }
 
@@ -509,9 +630,14 @@ hipError_t launch_arm(Grid& g, hipStream_t st) {
 }
 
 hipError_t launch_errors(Grid& g, hipStream_t st) {
-    // grid-stride over the device-side leaf count; sized for the capacity
-    const int blocks = grid_for(g.cap_l, 4096);
-    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_errors<DD>, dim3(blocks), dim3(256), 0, st, g.dev));
+    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_errors<DD>, dim3(kErrBlocks), dim3(256), 0, st, g.dev));
+    hipLaunchKernelGGL(k_err_reduce, dim3(1), dim3(1024), 0, st, g.dev, kErrBlocks);
+    return hipGetLastError();
+}
+
+hipError_t launch_errors_stale(Grid& g, hipStream_t st) {
+    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_errors_stale<DD>, dim3(kErrBlocks), dim3(256), 0, st, g.dev));
+    hipLaunchKernelGGL(k_err_reduce, dim3(1), dim3(1024), 0, st, g.dev, kErrBlocks);
     return hipGetLastError();
 }
 

@@ -53,11 +53,14 @@ hipError_t dalloc(T** p, size_t n) {
 
 void free_grid(Grid& g) {
     mdas::DevGrid& d = g.dev;
-    void* ptrs[] = {d.coords, d.values, d.idx,  d.bmn,    d.bmx,    d.err,         d.seed,
-                    d.flags,  d.part,   d.part_off, d.cnt, d.pstats, g.keys, g.keys_alt,
-                    g.tile_start, g.tile_end, g.rs_hist, g.d_scalar};
+    void* ptrs[] = {d.coords,   d.values, d.leaf,     d.err,  d.seed,       d.flags,
+                    d.split_list, d.part, d.part_off, d.epart, d.rrec,      d.cnt,
+                    d.pstats,   g.keys,   g.keys_alt, g.tile_start, g.tile_end, g.rs_hist,
+                    g.d_scalar};
     for (void* p : ptrs)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
+    for (hipEvent_t e : g.evpool) (void)hipEventDestroy(e);
+    g.evpool.clear();
     d = mdas::DevGrid{};
     g.keys = g.keys_alt = nullptr;
     g.tile_start = g.tile_end = nullptr;
@@ -162,14 +165,14 @@ int mdas_create(const mdas_config* cfg, mdas_handle* out) {
     const size_t cs = (size_t)g.cap_s, cl = (size_t)g.cap_l;
     if (e == hipSuccess) e = dalloc(&d.coords, cs * D);
     if (e == hipSuccess) e = dalloc(&d.values, cs * 3);
-    if (e == hipSuccess) e = dalloc(&d.idx, cl);
-    if (e == hipSuccess) e = dalloc(&d.bmn, cl * D);
-    if (e == hipSuccess) e = dalloc(&d.bmx, cl * D);
+    if (e == hipSuccess) e = dalloc(&d.leaf, cl * mdas::leaf_quads(D));
     if (e == hipSuccess) e = dalloc(&d.err, cl);
     if (e == hipSuccess) e = dalloc(&d.seed, cl);
     if (e == hipSuccess) e = dalloc(&d.flags, cl);
+    if (e == hipSuccess) e = dalloc(&d.split_list, cl);
     if (e == hipSuccess) e = dalloc(&d.part, (size_t)mdas::kChunkBlocks);
     if (e == hipSuccess) e = dalloc(&d.part_off, (size_t)mdas::kChunkBlocks);
+    if (e == hipSuccess) e = dalloc(&d.epart, (size_t)mdas::kErrBlocks);
     if (e == hipSuccess) e = dalloc(&d.cnt, 1);
     if (e == hipSuccess) e = dalloc(&d.pstats, (size_t)mdas::kMaxPassStats);
     if (e == hipSuccess) e = dalloc(&g.d_scalar, 64);
@@ -219,8 +222,9 @@ int mdas_initial_sampling(mdas_handle h, void* stream) {
         HIP_TRY(hipEventCreate(&e1));
         HIP_TRY(hipEventRecord(e0, st));
     }
-    // seeds of leaves that do not exist yet must read as 0 (kdgrid.cu:637)
-    HIP_TRY(mdas::launch_fill_i32((int*)g.dev.seed, 0, g.cap_l, st));
+    // The reference zeroes every seed here (kdgrid.cu:637) so that a leaf
+    // created later starts from tea4(index, 0); a split writes seed 0 into
+    // the new leaf's slot, which gives the same result without the fill.
     HIP_TRY(mdas::launch_initial(g, L0, st));
     g.initialized = true;
     g.host.n_leaves = L0;
@@ -311,7 +315,8 @@ int mdas_sampling_pass(mdas_handle h, void* stream) {
 
 static int check_image(Grid& g, int w, int hgt) {
     if (w <= 0 || hgt <= 0) return fail(MDAS_ERR_INVALID, "image size must be positive");
-    if ((long long)w * hgt > 0x7fffffffLL) return fail(MDAS_ERR_INVALID, "image too large");
+    if ((long long)w * hgt > 0x7fffffffLL || w > 65535 || hgt > 65535)
+        return fail(MDAS_ERR_INVALID, "image too large (at most 65535 pixels per side)");
     if (!g.initialized) return fail(MDAS_ERR_STATE, "InitialSampling has not been called");
     return MDAS_OK;
 }
@@ -401,18 +406,16 @@ int mdas_download_state(mdas_handle h, float* coords, float* values, int* idx, f
     const int D = g.D;
     if (coords) HIP_TRY(hipMemcpy(coords, g.dev.coords, sizeof(float) * N * D, hipMemcpyDeviceToHost));
     if (values) HIP_TRY(hipMemcpy(values, g.dev.values, sizeof(float) * N * 3, hipMemcpyDeviceToHost));
-    if (idx) HIP_TRY(hipMemcpy(idx, g.dev.idx, sizeof(int) * 4 * L, hipMemcpyDeviceToHost));
     if (err) HIP_TRY(hipMemcpy(err, g.dev.err, sizeof(float) * L, hipMemcpyDeviceToHost));
     if (seed) HIP_TRY(hipMemcpy(seed, g.dev.seed, sizeof(uint32_t) * L, hipMemcpyDeviceToHost));
-    if (box) {
-        std::vector<float> mn(L), mx(L);
-        for (int d = 0; d < D; ++d) {
-            HIP_TRY(hipMemcpy(mn.data(), g.dev.bmn + (size_t)d * g.cap_l, sizeof(float) * L, hipMemcpyDeviceToHost));
-            HIP_TRY(hipMemcpy(mx.data(), g.dev.bmx + (size_t)d * g.cap_l, sizeof(float) * L, hipMemcpyDeviceToHost));
-            for (size_t i = 0; i < L; ++i) {
-                box[i * 2 * D + d] = mn[i];
-                box[i * 2 * D + D + d] = mx[i];
-            }
+    if (idx || box) {
+        const int Q = mdas::leaf_quads(D);
+        std::vector<float> rec(L * 4 * Q);
+        HIP_TRY(hipMemcpy(rec.data(), g.dev.leaf, sizeof(float) * rec.size(), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < L; ++i) {
+            const float* r = &rec[i * 4 * Q];
+            if (idx) memcpy(idx + 4 * i, r, 16);
+            if (box) memcpy(box + i * 2 * D, r + 4, sizeof(float) * 2 * D);
         }
     }
     return MDAS_OK;
@@ -434,26 +437,21 @@ int mdas_upload_state(mdas_handle h, int n, int l, const float* coords, const fl
         memcpy(s.data(), seed, sizeof(uint32_t) * (size_t)l);
         HIP_TRY(hipMemcpy(g.dev.seed, s.data(), sizeof(uint32_t) * s.size(), hipMemcpyHostToDevice));
     }
-    if (idx) {
-        HIP_TRY(hipMemcpy(g.dev.idx, idx, sizeof(int) * 4 * (size_t)l, hipMemcpyHostToDevice));
+    if (idx || box) {
+        if (!idx || !box) return fail(MDAS_ERR_INVALID, "slots and boxes are uploaded together");
+        const int Q = mdas::leaf_quads(D);
+        std::vector<float> rec((size_t)l * 4 * Q, 0.0f);
         std::vector<unsigned char> f((size_t)l);
         for (int i = 0; i < l; ++i) {
+            float* r = &rec[(size_t)i * 4 * Q];
+            memcpy(r, idx + 4 * (size_t)i, 16);
+            memcpy(r + 4, box + (size_t)i * 2 * D, sizeof(float) * 2 * D);
             int c = 0;
             for (int q = 0; q < 4; ++q) c += idx[4 * i + q] >= 0 ? 1 : 0;
             f[i] = (unsigned char)c;
         }
+        HIP_TRY(hipMemcpy(g.dev.leaf, rec.data(), sizeof(float) * rec.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(g.dev.flags, f.data(), f.size(), hipMemcpyHostToDevice));
-    }
-    if (box) {
-        std::vector<float> mn((size_t)l), mx((size_t)l);
-        for (int d = 0; d < D; ++d) {
-            for (int i = 0; i < l; ++i) {
-                mn[i] = box[(size_t)i * 2 * D + d];
-                mx[i] = box[(size_t)i * 2 * D + D + d];
-            }
-            HIP_TRY(hipMemcpy(g.dev.bmn + (size_t)d * g.cap_l, mn.data(), sizeof(float) * l, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(g.dev.bmx + (size_t)d * g.cap_l, mx.data(), sizeof(float) * l, hipMemcpyHostToDevice));
-        }
     }
     mdas::DevCounters c{};
     c.n_samples = n;
@@ -636,7 +634,6 @@ int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream)
     g.evnext = 0;
     g.t_recon = g.t_sort = 0.0;
     mdas::kbeg(g, st, "k_initial");
-    HIP_TRY(mdas::launch_fill_i32((int*)g.dev.seed, 0, g.cap_l, st));
     HIP_TRY(mdas::launch_initial(g, L0, st));
     HIP_TRY(mdas::launch_set_limit(g, (int)limit, st));
     mdas::kend(g, st);
@@ -649,7 +646,7 @@ int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream)
         for (int b = 0; b < B; ++b) {
             mdas::kbeg(g, st, "k_errors");
             HIP_TRY(mdas::launch_pass_begin(g, st));
-            HIP_TRY(mdas::launch_errors(g, st));
+            HIP_TRY(mdas::launch_errors_stale(g, st));
             mdas::kend(g, st);
             mdas::kbeg(g, st, "k_select");
             HIP_TRY(mdas::launch_select(g, st));
@@ -706,41 +703,44 @@ int mdas_get_pass_stats(mdas_handle h, int* out, int max_rows) {
 }
 
 // Algorithmic bytes of each kernel over the last run + reconstruction: the
-// bytes the algorithm must move with this layout (FP32 / int32 values, int4
-// slots, SoA boxes of 8*D bytes, one flag byte), counted once.
+// bytes the algorithm must move with this layout, each counted once: FP32
+// coordinates (4*D per sample) and values (12), leaf records of LB = 48 or 64
+// bytes (slots + box, read or written whole), 4-byte errors and LCG states,
+// 1-byte flags, 8-byte keys, 48-byte reconstruction records, float4 pixels.
 static double kernel_bytes(const Grid& g, const std::string& k) {
     const double D = g.D;
+    const double LB = 16.0 * mdas::leaf_quads(g.D);
     const double L0 = (double)(1LL << (g.b * g.D + 2 * g.e));
     double b = 0.0;
-    if (k == "k_initial") return L0 * (16.0 * D + 16.0 + 8.0 * D + 4.0 + 4.0 + 1.0) + 4.0 * g.cap_l;
+    if (k == "k_initial") return L0 * (16.0 * D + LB + 4.0 + 4.0 + 1.0);
     if (k == "k_eval") {
         double n = (double)g.host.n_samples;
         return n * (4.0 * D + 12.0);
     }
     for (const auto& p : g.pstats_host) {
-        if (k == "k_errors")
-            b += 4.0 * p.L + (double)p.stale * (16.0 + 8.0 * D + 4.0) + 12.0 * p.stale_vals;
-        else if (k == "k_select")
+        if (k == "k_errors")  // stale leaves: record, values, error; half of them via the list
+            b += (double)p.stale * (LB + 4.0 + 2.0) + 12.0 * p.stale_vals;
+        else if (k == "k_select")  // error, state, flags read; state, flags written
             b += 14.0 * p.L;
         else if (k == "k_scan_counts")
-            b += 2.0 * 8.0 * mdas::kChunkBlocks;
+            b += (16.0 + 8.0) * mdas::kChunkBlocks;
         else if (k == "k_place")
-            b += 1.0 * p.L +
-                 (double)p.sel * (4.0 + 8.0 * D + 16.0 + 4.0 * D + 16.0 + 4.0 + 1.0) +
-                 4.0 * D * p.selcnt + (double)p.split * (8.0 * D + 4.0 + 16.0 + 8.0 + 4.0 + 1.0);
+            b += 1.0 * p.L +                                              // flags
+                 (double)p.sel * (4.0 + LB + 4.0 * D + 4.0 + 1.0) +       // state, record, new sample
+                 4.0 * D * p.selcnt +                                     // the leaf's samples
+                 (double)(p.sel - p.split) * 16.0 +                       // slots of an insert
+                 (double)p.split * (2.0 * LB + 8.0 + 4.0 + 2.0 + 4.0);    // two records, errors, ...
     }
     const double L = (double)g.host.n_leaves, N = (double)g.host.n_samples;
     const double K = (double)g.last_n_keys;
-    if (k == "k_recon_count") return L * 16.0;
-    if (k == "k_scan_keys") return 2.0 * 8.0 * mdas::kChunkBlocks;
+    if (k == "k_recon_prep") return L * (LB + 48.0) + 12.0 * N;
+    if (k == "k_scan_keys") return (16.0 + 8.0) * mdas::kChunkBlocks;
     if (k == "k_recon_emit") return L * 16.0 + K * 8.0;
     if (k == "k_rs_hist") return K * 8.0 * g.recon_sort_passes;
-    if (k == "k_rs_scan") return 8.0 * 256 * 1024 * g.recon_sort_passes;
+    if (k == "k_rs_scan") return 3.0 * 4.0 * 256 * 1024 * g.recon_sort_passes;
     if (k == "k_rs_scatter") return K * 16.0 * g.recon_sort_passes;
     if (k == "k_tile_ranges") return K * 8.0;
-    if (k == "k_gather")
-        return K * (8.0 + 8.0 * D + 16.0) + (L > 0 ? 12.0 * N * (K / L) : 0.0) +
-               16.0 * (double)g.recon_pixels;
+    if (k == "k_gather") return K * (8.0 + 48.0) + 16.0 * (double)g.recon_pixels;
     return b;
 }
 
@@ -779,16 +779,62 @@ int mdas_get_phase_times(mdas_handle h, mdas_phase_times* o) {
     o->bytes_adaptive = kernel_bytes(g, "k_select") + kernel_bytes(g, "k_scan_counts") +
                         kernel_bytes(g, "k_place");
     double br = 0.0;
-    for (const char* k : {"k_recon_count", "k_scan_keys", "k_recon_emit", "k_rs_hist", "k_rs_scan",
+    for (const char* k : {"k_recon_prep", "k_scan_keys", "k_recon_emit", "k_rs_hist", "k_rs_scan",
                           "k_rs_scatter", "k_tile_ranges", "k_gather"})
         br += kernel_bytes(g, k);
     o->bytes_recon = br;
     return MDAS_OK;
 }
 
+// Times the reconstruction of the current state with every candidate tile
+// shape (after one untimed warm-up each) and keeps the fastest.  Results are
+// identical for every shape; only the speed differs.
+int mdas_autotune_recon(mdas_handle h, float* pixels, int w, int hgt, void* stream, int* best_tw,
+                        int* best_th) {
+    if (!h || !pixels) return fail(MDAS_ERR_INVALID, "null argument");
+    Grid& g = h->g;
+    int rc = check_image(g, w, hgt);
+    if (rc) return rc;
+    static const int cand[][2] = {{8, 8}, {16, 4}, {4, 4}, {8, 2}, {16, 1}};
+    hipStream_t st = S(stream);
+    const bool timing = g.timing;
+    g.timing = false;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    int btw = g.tile_w, bth = g.tile_h;
+    for (const auto& c : cand) {
+        g.tile_w = c[0];
+        g.tile_h = c[1];
+        HIP_TRY(mdas::recon_prepare(g, w, hgt, st));
+        HIP_TRY(mdas::launch_integrate_gather(g, pixels, w, hgt, st, nullptr, nullptr));
+        HIP_TRY(hipEventRecord(e0, st));
+        HIP_TRY(mdas::recon_prepare(g, w, hgt, st));
+        HIP_TRY(mdas::launch_integrate_gather(g, pixels, w, hgt, st, nullptr, nullptr));
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        const float ms = elapsed(e0, e1);
+        if (ms < best) {
+            best = ms;
+            btw = c[0];
+            bth = c[1];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g.tile_w = btw;
+    g.tile_h = bth;
+    g.timing = timing;
+    if (best_tw) *best_tw = btw;
+    if (best_th) *best_th = bth;
+    return MDAS_OK;
+}
+
 int mdas_set_recon_tile(mdas_handle h, int tw, int th) {
     if (!h) return fail(MDAS_ERR_INVALID, "null handle");
-    if (tw <= 0 || th <= 0 || tw * th != 64) return fail(MDAS_ERR_INVALID, "tile must hold 64 pixels");
+    if (tw <= 0 || th <= 0 || (tw * th != 64 && tw * th != 16))
+        return fail(MDAS_ERR_INVALID, "a reconstruction tile holds 16 or 64 pixels");
     h->g.tile_w = tw;
     h->g.tile_h = th;
     return MDAS_OK;

@@ -26,7 +26,7 @@ struct DevCounters {
     int sel_cnt;         // samples held by the leaves selected in this pass
     int stale;           // leaves whose error was recomputed in this pass
     int stale_vals;      // sample values read for them
-    int pad;
+    unsigned carry_bits; // max error of the leaves the last adaptive pass did not split
 };
 
 // Per-pass statistics, one row per adaptive pass (for algorithmic byte counts):
@@ -44,20 +44,29 @@ struct KRec {
 };
 
 static const int kChunkBlocks = 2048;   // workgroups of the chunked leaf kernels
+static const int kErrBlocks = 1024;     // workgroups of the error kernels
 static const int kMaxPassStats = 4096;
+
+// Float4 quads of one leaf record: the int4 of sample slots, then the box
+// (mn[0..D-1], mx[0..D-1]) padded to 16 bytes -- 48 bytes for D = 3, 4 and
+// 64 bytes for D = 5, 6, so that a leaf read at a random index touches one
+// cache-line segment.
+__host__ __device__ constexpr int leaf_quads(int D) { return 1 + (2 * D + 3) / 4; }
 
 // Plain-pointer view of the device state passed to kernels by value.
 struct DevGrid {
-    float* coords;         // [cap_s * D]  AoS
+    float* coords;         // [cap_s * D]  AoS (the integrand contract)
     float* values;         // [cap_s * 3]  float3 AoS
-    int4* idx;             // [cap_l]      sample slots, ~D = empty
-    float* bmn;            // [D * cap_l]  box min, SoA by dimension
-    float* bmx;            // [D * cap_l]  box max, SoA by dimension
+    float4* leaf;          // [cap_l * leaf_quads(D)]  slots (~D = empty) + box
     float* err;            // [cap_l]      < 0 = stale
     unsigned* seed;        // [cap_l]      LCG state, 0 = unset
     unsigned char* flags;  // [cap_l]      bits 0-2 count, 3 selected, 4 split
-    int2* part;            // [kChunkBlocks] per-workgroup counts
-    int2* part_off;        // [kChunkBlocks] exclusive prefix of part
+    int* split_list;       // [cap_l]      leaves split by the last pass, in leaf order
+    int4* part;            // [kChunkBlocks] per-workgroup (selected, split, samples, carry)
+    int2* part_off;        // [kChunkBlocks] exclusive prefix of (selected, split)
+    uint4* epart;          // [kErrBlocks] per-workgroup (max error bits, stale, values)
+    float4* rrec;          // [cap_l * 3]  reconstruction record per leaf
+    unsigned char* rcnt;   // [cap_l]      reconstruction tiles per leaf
     DevCounters* cnt;
     PassStat* pstats;
     int cap_s, cap_l;
@@ -131,6 +140,7 @@ inline void kend(Grid& g, hipStream_t st) {
 hipError_t launch_initial(Grid& g, int L0, hipStream_t st);
 hipError_t launch_reset_errors(Grid& g, int standalone, hipStream_t st);
 hipError_t launch_errors(Grid& g, hipStream_t st);
+hipError_t launch_errors_stale(Grid& g, hipStream_t st);
 hipError_t launch_adaptive(Grid& g, hipStream_t st);
 hipError_t launch_select(Grid& g, hipStream_t st);
 hipError_t launch_scan(Grid& g, hipStream_t st);

@@ -188,6 +188,13 @@ class KDGrid:
     def set_recon_tile(self, tw, th):
         check(_lib.load().mdas_set_recon_tile(self._h, tw, th))
 
+    def autotune_recon(self, pixels, width, height):
+        """Pick the fastest reconstruction tile for the current state; returns (tw, th)."""
+        tw, th = ctypes.c_int(), ctypes.c_int()
+        check(_lib.load().mdas_autotune_recon(self._h, pixels, width, height, self._s(),
+                                              ctypes.byref(tw), ctypes.byref(th)))
+        return tw.value, th.value
+
     def download_state(self):
         """Host copies: coords (N,D), values (N,3), idx (L,4), box (L,2D), err (L), seed (L)."""
         lib = _lib.load()

@@ -148,7 +148,7 @@ def test_run_loop_bit_exact(mdas, kind, dim, b, e, W, H, spp, alpha, cand):
 
 
 @pytest.mark.parametrize("kind,dim,b,e,W,H,spp,alpha,cand", RUNS[:4])
-@pytest.mark.parametrize("tile", [(8, 8), (16, 4)])
+@pytest.mark.parametrize("tile", [(8, 8), (16, 4), (4, 4), (8, 2)])
 def test_integrate_bit_exact(mdas, kind, dim, b, e, W, H, spp, alpha, cand, tile):
     _say(f"integrate kind={kind} dim={dim} tile={tile}")
     g, o, sc, target = _make(mdas, kind, dim, b, e, W, H, spp, alpha, cand)
@@ -173,6 +173,19 @@ def test_integrate_other_resolution(mdas):
         _say(f"integrate {W}x{H}")
         px = g.integrate_image(W, H)
         assert np.array_equal(px.view(np.uint32), o.integrate(W, H).view(np.uint32))
+
+
+def test_autotune_keeps_result(mdas):
+    from pmdas_amd import hip
+    g, o, sc, target = _make(mdas, 3, 5, 1, 2, 80, 48, 16, 0.25, 4)
+    g.Run(sc, target)
+    o.run(sc.params_bytes(), target)
+    buf = hip.DeviceBuffer(80 * 48 * 16)
+    tw, th = g.autotune_recon(buf.ptr, 80, 48)
+    assert tw * th in (16, 64)
+    buf.free()
+    px = g.integrate_image(80, 48)
+    assert np.array_equal(px.view(np.uint32), o.integrate(80, 48).view(np.uint32))
 
 
 def test_convert_to_bytes_bit_exact(mdas):
