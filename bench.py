@@ -99,6 +99,63 @@ def run_gpu(name, steps, warmup, log=print):
     return res
 
 
+def batch_images(n=32, seed=7):
+    """n images with pixel counts log-uniform in [256^2, 2048^2] (16:9-ish),
+    extraImgBits chosen per image so the initial grid holds about 4 spp."""
+    import math
+    import random
+    rng = random.Random(seed)
+    out = []
+    for i in range(n):
+        p = 2.0 ** rng.uniform(16.0, 22.0)
+        W = max(64, int(round(math.sqrt(p * 16.0 / 9.0))))
+        H = max(64, int(round(p / W)))
+        e = int(math.floor(0.5 * math.log2(W * H / 32.0) + 0.5))
+        out.append(dict(dim=5, kind=3, W=W, H=H, spp=32, b=1, e=e, alpha=0.25, cand=4, seed=100 + i))
+    return out
+
+
+def run_batch(steps, log=print):
+    """Config 3: 32 images of uneven resolution, 5D, 32 spp, advanced as one
+    batch replayed from a captured hipGraph, then reconstructed one by one."""
+    from pmdas_amd import hip, kdgrid
+    imgs = batch_images()
+    grids, scenes, targets, bufs = [], [], [], []
+    for w in imgs:
+        target, cap = capacity(w)
+        grids.append(kdgrid.KDGrid(w["dim"], cap, w["cand"], w["b"], w["e"], w["alpha"],
+                                   float(w["W"]), float(w["H"])))
+        scenes.append(kdgrid.Scene(w["kind"], w["seed"], w["dim"], float(w["W"]), float(w["H"])))
+        targets.append(target)
+        bufs.append(hip.DeviceBuffer(w["W"] * w["H"] * 16))
+    batch = kdgrid.Batch(grids, scenes)
+    times = []
+    for i in range(steps + 1):
+        hip.synchronize()
+        t0 = time.perf_counter()
+        batch.run(targets, use_graph=True)
+        for g, w, b in zip(grids, imgs, bufs):
+            g.Integrate(b.ptr, w["W"], w["H"])
+        hip.synchronize()
+        t1 = time.perf_counter()
+        if i > 0:
+            times.append((t1 - t0) * 1e3)
+        log(f"[bench] batch of {len(imgs)}: {'warmup' if i == 0 else 'step'} "
+            f"{(t1 - t0) * 1e3:.1f} ms")
+    n = sum(g.GetNumberOfSamples() for g in grids)
+    ms = sum(times) / len(times)
+    res = dict(images=len(imgs), pixels=sum(w["W"] * w["H"] for w in imgs), samples=n,
+               ms_per_step=round(ms, 2), e2e_msamples_per_s=round(n / (ms * 1e3), 1),
+               passes_max=max(g.pass_stats().shape[0] for g in grids),
+               note="wall time of batch run (with the integrand) + 32 reconstructions")
+    batch.close()
+    for b in bufs:
+        b.free()
+    for g in grids:
+        g.close()
+    return res
+
+
 def roofline(res, traffic_file):
     """Dominant MDAS kernel (integrand excluded): algorithmic bytes / its time."""
     ks = {k: v for k, v in res["kernels"].items() if k != "k_eval"}
@@ -171,6 +228,7 @@ def main():
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cpu-shrink", type=int, default=0,
                     help="run the CPU baseline on a 2^-k per-axis version of the workload")
     a = ap.parse_args()
@@ -227,6 +285,8 @@ def main():
                              "unit": "Msamples/s", "mdas_ms_per_step": round(r2["mdas_ms"], 3),
                              "e2e_msamples_per_s": round(r2["e2e_msamples_per_s"], 2),
                              "samples": r2["samples"], "passes": r2["passes"]}
+    if not a.no_batch:
+        line["batch"] = run_batch(max(1, min(2, a.steps)), log)
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_shrink, log)
     else:

@@ -170,6 +170,20 @@ int mdas_set_values(mdas_handle h, int offset, int count, const float* host_valu
  * last mdas_integrate; *n = number of keys. */
 int mdas_get_recon_keys(mdas_handle h, uint64_t* out, long long capacity, long long* n);
 
+/* ---- batches ------------------------------------------------------------------
+ * Many grids (images of uneven resolution; dimensions are grouped) advanced
+ * together: one launch per kernel and pass for every image of a dimension,
+ * the block of passes captured once into a hipGraph and replayed until every
+ * image meets its stopping rule.  Each image's result equals mdas_run on it
+ * alone.  The grids and scenes must outlive the batch. */
+typedef struct mdas_batch* mdas_batch_handle;
+int mdas_batch_create(mdas_handle* grids, mdas_scene_handle* scenes, int n, mdas_batch_handle* out);
+int mdas_batch_destroy(mdas_batch_handle b);
+/* targets[i]: sample target of image i; use_graph: replay a captured graph */
+int mdas_batch_run(mdas_batch_handle b, const long long* targets, void* stream, int use_graph);
+/* wall time of the last run (ms), blocks of passes run, dimension groups */
+int mdas_batch_info(mdas_batch_handle b, double* wall_ms, int* pass_blocks, int* groups);
+
 /* ---- configuration files ----------------------------------------------------
  * Reader of the reference's ".env" files (SDK/sutil/Environment.cpp:593-792)
  * for the Sampler / Mdas / Film options (optixMotionBlur.cpp:107-134) plus a

@@ -130,6 +130,11 @@ SIGNATURES = {
     "mdas_upload_state": (_I, [_H, _I, _I, _P, _P, _P, _P, _P, _P, _F]),
     "mdas_set_values": (_I, [_H, _I, _I, _P]),
     "mdas_get_recon_keys": (_I, [_H, _P, _LL, ctypes.POINTER(_LL)]),
+    "mdas_batch_create": (_I, [_P, _P, _I, ctypes.POINTER(_H)]),
+    "mdas_batch_destroy": (_I, [_H]),
+    "mdas_batch_run": (_I, [_H, _P, _P, _I]),
+    "mdas_batch_info": (_I, [_H, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I),
+                             ctypes.POINTER(_I)]),
     "mdas_env_read": (_I, [ctypes.c_char_p, ctypes.POINTER(MdasEnv)]),
     "mdas_env_last_error": (ctypes.c_char_p, []),
     "mdas_env_config": (_I, [ctypes.POINTER(MdasEnv), _LL, ctypes.POINTER(MdasConfig)]),

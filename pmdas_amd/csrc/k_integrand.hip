@@ -10,11 +10,11 @@
 
 namespace mdas {
 
-__global__ __launch_bounds__(256) void k_eval(const float* __restrict__ coords,
-                                              float* __restrict__ values, int dim,
-                                              const mscene::Params* __restrict__ P,
-                                              int first, int count, const DevCounters* cnt,
-                                              int dev_range) {
+__device__ __forceinline__ void eval_body(const float* __restrict__ coords,
+                                          float* __restrict__ values, int dim,
+                                          const mscene::Params* __restrict__ P, int first,
+                                          int count, const DevCounters* cnt, int dev_range, int blk,
+                                          int nblk) {
     __shared__ float obj[mscene::MAX_OBJ * mscene::OBJ_STRIDE];
     int beg = first, end = first + count;
     if (dev_range) {
@@ -26,7 +26,7 @@ __global__ __launch_bounds__(256) void k_eval(const float* __restrict__ coords,
     const int nobj = P->n_obj;
     for (int k = threadIdx.x; k < nobj * mscene::OBJ_STRIDE; k += 256) obj[k] = P->obj[k];
     __syncthreads();
-    for (int i = beg + blockIdx.x * 256 + threadIdx.x; i < end; i += gridDim.x * 256) {
+    for (int i = beg + blk * 256 + threadIdx.x; i < end; i += nblk * 256) {
         float x[6];
         const float* c = coords + (size_t)i * dim;
         for (int d = 0; d < dim; ++d) x[d] = c[d];
@@ -38,6 +38,20 @@ __global__ __launch_bounds__(256) void k_eval(const float* __restrict__ coords,
     }
 }
 
+__global__ __launch_bounds__(256) void k_eval(const float* __restrict__ coords,
+                                              float* __restrict__ values, int dim,
+                                              const mscene::Params* __restrict__ P,
+                                              int first, int count, const DevCounters* cnt,
+                                              int dev_range) {
+    eval_body(coords, values, dim, P, first, count, cnt, dev_range, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(256) void k_eval_b(const DevGrid* grids, const void* const* params) {
+    const DevGrid& a = grids[blockIdx.y];
+    eval_body(a.coords, a.values, a.D, (const mscene::Params*)params[blockIdx.y], 0, 0, a.cnt, 1,
+              blockIdx.x, gridDim.x);
+}
+
 hipError_t launch_eval(Grid& g, const void* dparams, int first, int count, bool dev_range,
                        hipStream_t st) {
     long long n = dev_range ? (long long)g.cap_s : (long long)count;
@@ -47,6 +61,12 @@ hipError_t launch_eval(Grid& g, const void* dparams, int first, int count, bool 
     hipLaunchKernelGGL(k_eval, dim3((unsigned)blocks), dim3(256), 0, st, g.dev.coords,
                        g.dev.values, g.D, (const mscene::Params*)dparams, first, count,
                        (const DevCounters*)g.dev.cnt, dev_range ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_eval_b(const DevGrid* grids, const void* const* params, int n, int blocks,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(k_eval_b, dim3(blocks, n), dim3(256), 0, st, grids, params);
     return hipGetLastError();
 }
 

@@ -532,9 +532,23 @@ static ReconGeo make_geo(const Grid& g, int W, int H) {
 hipError_t recon_prepare(Grid& g, int W, int H, hipStream_t st) {
     ReconGeo G = make_geo(g, W, H);
     g.recon_pixels = (long long)W * H;
-    if (!g.dev.rrec) {
-        hipError_t e0 = hipMalloc(&g.dev.rrec, sizeof(float4) * 3 * (size_t)g.cap_l);
+    {
+        // records for the current leaves (the count is read back first)
+        int L = 0;
+        hipError_t e0 = hipMemcpyAsync(&L, &g.dev.cnt->n_leaves, sizeof(int), hipMemcpyDeviceToHost, st);
+        if (e0 == hipSuccess) e0 = hipStreamSynchronize(st);
         if (e0 != hipSuccess) return e0;
+        if ((long long)L > g.rrec_cap) {
+            if (g.dev.rrec) (void)hipFree(g.dev.rrec);
+            g.dev.rrec = nullptr;
+            long long cap = std::min<long long>((long long)g.cap_l, (long long)L + L / 4 + 1024);
+            e0 = hipMalloc(&g.dev.rrec, sizeof(float4) * 3 * (size_t)cap);
+            if (e0 != hipSuccess) {
+                g.rrec_cap = 0;
+                return e0;
+            }
+            g.rrec_cap = cap;
+        }
     }
     kbeg(g, st, "k_recon_prep");
     MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_recon_prep<DD>, dim3(kChunkBlocks), dim3(256), 0, st,

@@ -22,9 +22,9 @@ namespace mdas {
 // workgroup's 256 leaves are one contiguous range, written through LDS so
 // that every store instruction covers 256 contiguous bytes.
 template <int D>
-__global__ __launch_bounds__(256) void k_initial(DevGrid a, int L0) {
+__device__ __forceinline__ void initial_body(const DevGrid& a, int L0, int blk, int nblk) {
     __shared__ float stage[256 * 4 * D];
-    for (int base = blockIdx.x * 256; base < L0; base += gridDim.x * 256) {
+    for (int base = blk * 256; base < L0; base += nblk * 256) {
         const int i = base + threadIdx.x;
         if (i < L0) {
             float off[D], ext[D];
@@ -54,8 +54,21 @@ __global__ __launch_bounds__(256) void k_initial(DevGrid a, int L0) {
     }
 }
 
+template <int D>
+__global__ __launch_bounds__(256) void k_initial(DevGrid a, int L0) {
+    initial_body<D>(a, L0, blockIdx.x, gridDim.x);
+}
+
+// Batched form: image blockIdx.y of the set, initial leaf count from its
+// configuration.
+template <int D>
+__global__ __launch_bounds__(256) void k_initial_b(const DevGrid* grids) {
+    const DevGrid& a = grids[blockIdx.y];
+    initial_body<D>(a, 1 << (a.b * D + 2 * a.e), blockIdx.x, gridDim.x);
+}
+
 // Counter set-up of InitialSampling (kdgrid.cu:639-642).
-__global__ void k_initial_counters(DevCounters* c, int L0) {
+__device__ __forceinline__ void initial_counters_body(DevCounters* c, int L0) {
     c->n_leaves = L0;
     c->n_samples = 4 * L0;
     c->n_leaves_old = 0;
@@ -72,6 +85,7 @@ __global__ void k_initial_counters(DevCounters* c, int L0) {
     c->stale_vals = 0;
     c->carry_bits = 0;
 }
+__global__ void k_initial_counters(DevCounters* c, int L0) { initial_counters_body(c, L0); }
 
 // ComputeErrors prologue (kdgrid.cu:684-685): g_error = 0.  A standalone call
 // also re-arms the pass kernels.
@@ -83,7 +97,7 @@ __global__ void k_reset_errors(DevCounters* c, int standalone) {
 // One pass of the device-side loop (optixMotionBlur.cpp:781): stop when
 // N >= min(target, max_samples).  gmax starts at the max error of the leaves
 // the previous pass did not split (their errors are not re-estimated).
-__global__ void k_pass_begin(DevCounters* c) {
+__device__ __forceinline__ void pass_begin_body(DevCounters* c) {
     if (c->done) return;
     if (c->n_samples >= c->limit) {
         c->done = 1;
@@ -91,8 +105,23 @@ __global__ void k_pass_begin(DevCounters* c) {
     }
     c->gmax_bits = c->passes == 0 ? 0u : c->carry_bits;
 }
+__global__ void k_pass_begin(DevCounters* c) { pass_begin_body(c); }
 
 __global__ void k_set_limit(DevCounters* c, int limit) { c->limit = limit; }
+
+// Batched counter kernels: one thread per image.
+__global__ void k_initial_counters_b(const DevGrid* grids, int n, const int* limits) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const DevGrid& a = grids[t];
+    DevCounters* c = a.cnt;
+    initial_counters_body(c, 1 << (a.b * a.D + 2 * a.e));
+    c->limit = limits[t];
+}
+__global__ void k_pass_begin_b(const DevGrid* grids, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) pass_begin_body(grids[t].cnt);
+}
 
 // Re-arms the pass kernels for a standalone AdaptiveSampling call.
 __global__ void k_arm(DevCounters* c) {
@@ -154,7 +183,7 @@ __device__ __forceinline__ float leaf_error(const DevGrid& a, int i, int* nvals)
 
 // Workgroup reduction of (max, stale count, values read) into epart[block].
 __device__ __forceinline__ void errors_epilogue(const DevGrid& a, float m, int n_stale, int n_vals,
-                                                float* wmax, int* wst) {
+                                                float* wmax, int* wst, int blk) {
     m = wave_max(m);
     n_stale = wave_sum(n_stale);
     n_vals = wave_sum(n_vals);
@@ -166,7 +195,7 @@ __device__ __forceinline__ void errors_epilogue(const DevGrid& a, float m, int n
     __syncthreads();
     if (threadIdx.x == 0) {
         const float b = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
-        a.epart[blockIdx.x] = make_uint4(__float_as_uint(b), (unsigned)(wst[0] + wst[1] + wst[2] + wst[3]),
+        a.epart[blk] = make_uint4(__float_as_uint(b), (unsigned)(wst[0] + wst[1] + wst[2] + wst[3]),
                                          (unsigned)(wst[4] + wst[5] + wst[6] + wst[7]), 0u);
     }
 }
@@ -194,7 +223,7 @@ __global__ __launch_bounds__(256) void k_errors(DevGrid a) {
         }
         m = fmaxf(m, e);
     }
-    errors_epilogue(a, m, n_stale, n_vals, wmax, wst);
+    errors_epilogue(a, m, n_stale, n_vals, wmax, wst, blockIdx.x);
 }
 
 // The same result inside the sampling loop without the sweep over all
@@ -203,7 +232,7 @@ __global__ __launch_bounds__(256) void k_errors(DevGrid a) {
 // leaves was carried by the selection kernel into gmax by k_pass_begin.  The
 // first pass after the initial grid re-estimates every leaf.
 template <int D>
-__global__ __launch_bounds__(256) void k_errors_stale(DevGrid a) {
+__device__ __forceinline__ void errors_stale_body(const DevGrid& a, int blk, int nblk) {
     __shared__ float wmax[4];
     __shared__ int wst[8];
     if (a.cnt->done) return;
@@ -213,7 +242,7 @@ __global__ __launch_bounds__(256) void k_errors_stale(DevGrid a) {
     const int L_old = a.cnt->n_leaves_old;
     float m = 0.0f;
     int n_stale = 0, n_vals = 0;
-    for (int t = blockIdx.x * 256 + threadIdx.x; t < n_items; t += gridDim.x * 256) {
+    for (int t = blk * 256 + threadIdx.x; t < n_items; t += nblk * 256) {
         const int i = first ? t : (t < nn ? a.split_list[t] : L_old + (t - nn));
         int n;
         const float e = leaf_error<D>(a, i, &n);
@@ -222,11 +251,20 @@ __global__ __launch_bounds__(256) void k_errors_stale(DevGrid a) {
         n_vals += n;
         m = fmaxf(m, e);
     }
-    errors_epilogue(a, m, n_stale, n_vals, wmax, wst);
+    errors_epilogue(a, m, n_stale, n_vals, wmax, wst, blk);
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_errors_stale(DevGrid a) {
+    errors_stale_body<D>(a, blockIdx.x, gridDim.x);
+}
+template <int D>
+__global__ __launch_bounds__(256) void k_errors_stale_b(const DevGrid* grids) {
+    errors_stale_body<D>(grids[blockIdx.y], blockIdx.x, gridDim.x);
 }
 
 // Folds the error partials into gmax and the per-pass statistics.
-__global__ __launch_bounds__(1024) void k_err_reduce(DevGrid a, int G) {
+__device__ __forceinline__ void err_reduce_body(const DevGrid& a, int G) {
     __shared__ unsigned sm[16];
     __shared__ int ss[16], sv[16];
     if (a.cnt->done) return;
@@ -263,6 +301,10 @@ __global__ __launch_bounds__(1024) void k_err_reduce(DevGrid a, int G) {
         a.cnt->stale_vals += s2;
     }
 }
+__global__ __launch_bounds__(1024) void k_err_reduce(DevGrid a, int G) { err_reduce_body(a, G); }
+__global__ __launch_bounds__(1024) void k_err_reduce_b(const DevGrid* grids, int G) {
+    err_reduce_body(grids[blockIdx.x], G);
+}
 
 // --------------------------------------------------------------- selection --
 // kdgrid.cu:316-326.  Per leaf: the seed (tea4(i,0) when unset), one draw u,
@@ -270,7 +312,7 @@ __global__ __launch_bounds__(1024) void k_err_reduce(DevGrid a, int G) {
 // Writes the advanced LCG state and the flags, and per workgroup the
 // (selected, split) counts for the allocation scan, the samples held by the
 // selected leaves, and the max error of the leaves that do not split.
-__global__ __launch_bounds__(256) void k_select(DevGrid a) {
+__device__ __forceinline__ void select_body(const DevGrid& a, int blk, int nblk) {
     __shared__ int red[12];
     __shared__ float rmax[4];
     if (a.cnt->done) return;
@@ -278,7 +320,7 @@ __global__ __launch_bounds__(256) void k_select(DevGrid a) {
     const float gmax = __uint_as_float(a.cnt->gmax_bits);
     const float alpha = a.alpha;
     int beg, end;
-    chunk_range(L, gridDim.x, blockIdx.x, beg, end);
+    chunk_range(L, nblk, blk, beg, end);
     int ns = 0, nl = 0, nc = 0;
     float carry = 0.0f;
     for (int i = beg + threadIdx.x; i < end; i += 256) {
@@ -309,14 +351,18 @@ __global__ __launch_bounds__(256) void k_select(DevGrid a) {
     __syncthreads();
     if (threadIdx.x == 0) {
         const float cm = fmaxf(fmaxf(rmax[0], rmax[1]), fmaxf(rmax[2], rmax[3]));
-        a.part[blockIdx.x] = make_int4(red[0] + red[1] + red[2] + red[3], red[4] + red[5] + red[6] + red[7],
-                                       red[8] + red[9] + red[10] + red[11], (int)__float_as_uint(cm));
+        a.part[blk] = make_int4(red[0] + red[1] + red[2] + red[3], red[4] + red[5] + red[6] + red[7],
+                                red[8] + red[9] + red[10] + red[11], (int)__float_as_uint(cm));
     }
+}
+__global__ __launch_bounds__(256) void k_select(DevGrid a) { select_body(a, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(256) void k_select_b(const DevGrid* grids) {
+    select_body(grids[blockIdx.y], blockIdx.x, gridDim.x);
 }
 
 // Exclusive scan of the per-workgroup counts (one workgroup of 1024 threads),
 // then the capacity check and the counter update (kdgrid.cu:753-759).
-__global__ __launch_bounds__(1024) void k_scan_counts(DevGrid a, int G) {
+__device__ __forceinline__ void scan_counts_body(const DevGrid& a, int G) {
     __shared__ int ls[16], ll[16], lc[16];
     __shared__ unsigned lm[16];
     if (a.cnt->done) return;
@@ -395,6 +441,10 @@ __global__ __launch_bounds__(1024) void k_scan_counts(DevGrid a, int G) {
         cn->stale = 0;
         cn->stale_vals = 0;
     }
+}
+__global__ __launch_bounds__(1024) void k_scan_counts(DevGrid a, int G) { scan_counts_body(a, G); }
+__global__ __launch_bounds__(1024) void k_scan_counts_b(const DevGrid* grids, int G) {
+    scan_counts_body(grids[blockIdx.x], G);
 }
 
 // ----------------------------------------------------- placement and split --
@@ -540,7 +590,7 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
 // in LDS; whenever the queue holds more than three waves' worth they are
 // placed with all lanes busy (the selection rate is typically 20-60 %).
 template <int D>
-__global__ __launch_bounds__(256) void k_place(DevGrid a) {
+__device__ __forceinline__ void place_body(const DevGrid& a, int blk, int nblk) {
     __shared__ int lds[8];
     __shared__ int q_leaf[1024], q_k[1024], q_m[1024];
     __shared__ unsigned char q_cnt[1024];
@@ -548,8 +598,8 @@ __global__ __launch_bounds__(256) void k_place(DevGrid a) {
     const int L = a.cnt->n_leaves_old;
     const int N0 = a.cnt->n_samples_old;
     int beg, end;
-    chunk_range(L, gridDim.x, blockIdx.x, beg, end);
-    int2 base = a.part_off[blockIdx.x];
+    chunk_range(L, nblk, blk, beg, end);
+    int2 base = a.part_off[blk];
     int qn = 0;
     for (int t0 = beg; t0 < end; t0 += 256) {
         const int i = t0 + threadIdx.x;
@@ -575,6 +625,15 @@ __global__ __launch_bounds__(256) void k_place(DevGrid a) {
             __syncthreads();
         }
     }
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_place(DevGrid a) {
+    place_body<D>(a, blockIdx.x, gridDim.x);
+}
+template <int D>
+__global__ __launch_bounds__(256) void k_place_b(const DevGrid* grids) {
+    place_body<D>(grids[blockIdx.y], blockIdx.x, gridDim.x);
 }
 
 // --------------------------------------------------------------- fills --
@@ -670,6 +729,27 @@ hipError_t launch_fill_i32(int* p, int v, long long n, hipStream_t st) {
 
 hipError_t launch_fill_f32(float* p, float v, long long n, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n)), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------ batched launchers --
+// `grids` is a device array of n DevGrid of one dimension D; every chunked
+// kernel gives each image `blocks` workgroups (<= kChunkBlocks).
+hipError_t launch_initial_b(int D, const DevGrid* grids, int n, int blocks, const int* limits,
+                            hipStream_t st) {
+    MDAS_DISPATCH_D(D, hipLaunchKernelGGL(k_initial_b<DD>, dim3(blocks, n), dim3(256), 0, st, grids));
+    hipLaunchKernelGGL(k_initial_counters_b, dim3((n + 63) / 64), dim3(64), 0, st, grids, n, limits);
+    return hipGetLastError();
+}
+
+hipError_t launch_pass_b(int D, const DevGrid* grids, int n, int blocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_pass_begin_b, dim3((n + 63) / 64), dim3(64), 0, st, grids, n);
+    const int eb = blocks < kErrBlocks ? blocks : kErrBlocks;
+    MDAS_DISPATCH_D(D, hipLaunchKernelGGL(k_errors_stale_b<DD>, dim3(eb, n), dim3(256), 0, st, grids));
+    hipLaunchKernelGGL(k_err_reduce_b, dim3(n), dim3(1024), 0, st, grids, eb);
+    hipLaunchKernelGGL(k_select_b, dim3(blocks, n), dim3(256), 0, st, grids);
+    hipLaunchKernelGGL(k_scan_counts_b, dim3(n), dim3(1024), 0, st, grids, blocks);
+    MDAS_DISPATCH_D(D, hipLaunchKernelGGL(k_place_b<DD>, dim3(blocks, n), dim3(256), 0, st, grids));
     return hipGetLastError();
 }
 

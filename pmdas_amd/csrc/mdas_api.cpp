@@ -14,19 +14,9 @@
 #include <string>
 #include <vector>
 
-#include "mdas_internal.hpp"
-#include "scenes.hpp"
+#include "mdas_handles.hpp"
 
 using mdas::Grid;
-
-struct mdas_grid {
-    Grid g;
-};
-
-struct mdas_scene {
-    mscene::Params host;
-    void* dev = nullptr;
-};
 
 namespace {
 
@@ -119,6 +109,11 @@ void collect_kernel_times(Grid& g) {
 
 }  // namespace
 
+namespace mdas {
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+int sync_grid(Grid& g, hipStream_t st) { return sync_counters(g, st); }
+}  // namespace mdas
+
 extern "C" {
 
 const char* mdas_last_error(void) { return t_err.c_str(); }
@@ -189,6 +184,7 @@ int mdas_create(const mdas_config* cfg, mdas_handle* out) {
         delete h;
         return fail(MDAS_ERR_HIP, std::string("device init failed: ") + hipGetErrorString(e));
     }
+    d.D = g.D;
     d.cap_s = g.cap_s;
     d.cap_l = g.cap_l;
     d.cand = g.cand;

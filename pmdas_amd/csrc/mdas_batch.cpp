@@ -1,0 +1,227 @@
+// Batched MDAS: many grids (images of uneven resolution, grouped by point
+// dimension) advanced together.  Every pass of a group is one launch per
+// kernel for all of its images (the kernels take a device array of grid
+// descriptors and read their image from blockIdx.y), and a block of passes is
+// captured once into a hipGraph and replayed until every image's device-side
+// stopping rule has fired.  Each image follows exactly the single-image
+// algorithm, so its result equals mdas_run on that image alone.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include "mdas_handles.hpp"
+
+namespace {
+
+struct Group {
+    int D = 0;
+    std::vector<int> members;           // batch indices
+    mdas::DevGrid* d_grids = nullptr;    // device copies of the descriptors
+    const void** d_params = nullptr;     // device array of scene parameter blocks
+    int* d_limits = nullptr;
+    int* d_status = nullptr;             // (done, status) per member
+    int blocks = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+#define B_TRY(expr)                                                                            \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return mdas::set_error(MDAS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+__global__ void k_batch_status(const mdas::DevGrid* grids, int n, int* out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        out[2 * t] = grids[t].cnt->done;
+        out[2 * t + 1] = grids[t].cnt->status;
+    }
+}
+
+}  // namespace
+
+struct mdas_batch {
+    std::vector<mdas_handle> grids;
+    std::vector<mdas_scene_handle> scenes;
+    std::vector<Group> groups;
+    hipStream_t own_stream = nullptr;
+    int passes_per_block = 4;
+    double last_ms = 0.0;
+    int last_blocks = 0;
+};
+
+extern "C" {
+
+int mdas_batch_create(mdas_handle* grids, mdas_scene_handle* scenes, int n, mdas_batch_handle* out) {
+    if (!grids || !scenes || !out || n <= 0) return mdas::set_error(MDAS_ERR_INVALID, "bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!grids[i] || !scenes[i]) return mdas::set_error(MDAS_ERR_INVALID, "null grid or scene");
+        if (scenes[i]->host.dim != grids[i]->g.D)
+            return mdas::set_error(MDAS_ERR_INVALID, "scene and grid dimensions differ at " + std::to_string(i));
+        for (int j = 0; j < i; ++j)
+            if (grids[j] == grids[i]) return mdas::set_error(MDAS_ERR_INVALID, "a grid appears twice");
+    }
+    mdas_batch* b = new mdas_batch;
+    b->grids.assign(grids, grids + n);
+    b->scenes.assign(scenes, scenes + n);
+    for (int D = 3; D <= 6; ++D) {
+        Group gr;
+        gr.D = D;
+        for (int i = 0; i < n; ++i)
+            if (grids[i]->g.D == D) gr.members.push_back(i);
+        if (gr.members.empty()) continue;
+        const int m = (int)gr.members.size();
+        // workgroups per image: about 8192 per launch in all, at least 64
+        gr.blocks = std::max(64, std::min(mdas::kChunkBlocks, 8192 / m));
+        std::vector<mdas::DevGrid> dg(m);
+        std::vector<const void*> dp(m);
+        for (int k = 0; k < m; ++k) {
+            dg[k] = grids[gr.members[k]]->g.dev;
+            dp[k] = scenes[gr.members[k]]->dev;
+        }
+        hipError_t e = hipMalloc(&gr.d_grids, sizeof(mdas::DevGrid) * m);
+        if (e == hipSuccess) e = hipMalloc((void**)&gr.d_params, sizeof(void*) * m);
+        if (e == hipSuccess) e = hipMalloc(&gr.d_limits, sizeof(int) * m);
+        if (e == hipSuccess) e = hipMalloc(&gr.d_status, sizeof(int) * 2 * m);
+        if (e == hipSuccess) e = hipMemcpy(gr.d_grids, dg.data(), sizeof(mdas::DevGrid) * m, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(gr.d_params, dp.data(), sizeof(void*) * m, hipMemcpyHostToDevice);
+        b->groups.push_back(gr);
+        if (e != hipSuccess) {
+            mdas_batch_destroy(b);
+            return mdas::set_error(MDAS_ERR_HIP, std::string("batch allocation: ") + hipGetErrorString(e));
+        }
+    }
+    *out = b;
+    return MDAS_OK;
+}
+
+int mdas_batch_destroy(mdas_batch_handle b) {
+    if (!b) return MDAS_OK;
+    (void)hipDeviceSynchronize();
+    for (Group& g : b->groups) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+        if (g.d_grids) (void)hipFree(g.d_grids);
+        if (g.d_params) (void)hipFree((void*)g.d_params);
+        if (g.d_limits) (void)hipFree(g.d_limits);
+        if (g.d_status) (void)hipFree(g.d_status);
+    }
+    if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
+    delete b;
+    return MDAS_OK;
+}
+
+// Runs every image of the batch to its target: initial grids, integrand,
+// then passes while N < min(target, max_samples) per image.  With use_graph
+// the block of passes is a captured hipGraph (a stream is created when the
+// caller passes the null stream, which cannot be captured).
+int mdas_batch_run(mdas_batch_handle b, const long long* targets, void* stream, int use_graph) {
+    if (!b || !targets) return mdas::set_error(MDAS_ERR_INVALID, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    if (use_graph && !st) {
+        if (!b->own_stream) B_TRY(hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking));
+        st = b->own_stream;
+    }
+    const int n = (int)b->grids.size();
+    for (int i = 0; i < n; ++i)
+        if (targets[i] < 0) return mdas::set_error(MDAS_ERR_INVALID, "negative target");
+    const auto t0 = std::chrono::steady_clock::now();
+    const int B = std::max(1, b->passes_per_block);
+    // initial grids and their integrand, one launch per kernel per group
+    std::vector<std::vector<int>> limits(b->groups.size());
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        Group& g = b->groups[gi];
+        const int m = (int)g.members.size();
+        limits[gi].resize(m);
+        for (int k = 0; k < m; ++k) {
+            const mdas::Grid& gg = b->grids[g.members[k]]->g;
+            limits[gi][k] = (int)std::min<long long>(targets[g.members[k]], gg.max_samples);
+        }
+        B_TRY(hipMemcpyAsync(g.d_limits, limits[gi].data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+        B_TRY(mdas::launch_initial_b(g.D, g.d_grids, m, g.blocks, g.d_limits, st));
+        B_TRY(mdas::launch_eval_b(g.d_grids, g.d_params, m, g.blocks, st));
+        for (int k = 0; k < m; ++k) b->grids[g.members[k]]->g.initialized = true;
+    }
+    std::vector<char> finished(b->groups.size(), 0);
+    int blocks_run = 0;
+    for (;;) {
+        for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+            if (finished[gi]) continue;
+            Group& g = b->groups[gi];
+            const int m = (int)g.members.size();
+            if (use_graph) {
+                if (!g.exec) {
+                    B_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                    hipError_t ce = hipSuccess;
+                    for (int p = 0; p < B && ce == hipSuccess; ++p) {
+                        ce = mdas::launch_pass_b(g.D, g.d_grids, m, g.blocks, st);
+                        if (ce == hipSuccess) ce = mdas::launch_eval_b(g.d_grids, g.d_params, m, g.blocks, st);
+                    }
+                    B_TRY(hipStreamEndCapture(st, &g.graph));
+                    B_TRY(ce);
+                    B_TRY(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+                }
+                B_TRY(hipGraphLaunch(g.exec, st));
+            } else {
+                for (int p = 0; p < B; ++p) {
+                    B_TRY(mdas::launch_pass_b(g.D, g.d_grids, m, g.blocks, st));
+                    B_TRY(mdas::launch_eval_b(g.d_grids, g.d_params, m, g.blocks, st));
+                }
+            }
+            hipLaunchKernelGGL(k_batch_status, dim3((m + 63) / 64), dim3(64), 0, st, g.d_grids, m, g.d_status);
+        }
+        ++blocks_run;
+        B_TRY(hipStreamSynchronize(st));
+        bool all = true;
+        for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+            if (finished[gi]) continue;
+            Group& g = b->groups[gi];
+            const int m = (int)g.members.size();
+            std::vector<int> s(2 * m);
+            B_TRY(hipMemcpy(s.data(), g.d_status, sizeof(int) * 2 * m, hipMemcpyDeviceToHost));
+            bool done = true;
+            for (int k = 0; k < m; ++k) {
+                if (s[2 * k + 1] != 0) {
+                    finished[gi] = 1;
+                    return mdas::set_error(s[2 * k + 1], "image " + std::to_string(g.members[k]) +
+                                                             ": adaptive pass would exceed max_samples");
+                }
+                done = done && s[2 * k] != 0;
+            }
+            finished[gi] = done ? 1 : 0;
+            all = all && done;
+        }
+        if (all) break;
+        if (blocks_run * B >= 20000) return mdas::set_error(MDAS_ERR_STATE, "batch exceeded 20000 passes");
+    }
+    for (int i = 0; i < n; ++i) {
+        mdas::Grid& gg = b->grids[i]->g;
+        int rc = mdas::sync_grid(gg, st);
+        if (rc) return rc;
+        gg.passes = gg.host.passes;
+        gg.pstats_host.assign((size_t)std::min(gg.passes, mdas::kMaxPassStats), mdas::PassStat{});
+        if (!gg.pstats_host.empty())
+            B_TRY(hipMemcpy(gg.pstats_host.data(), gg.dev.pstats,
+                            sizeof(mdas::PassStat) * gg.pstats_host.size(), hipMemcpyDeviceToHost));
+    }
+    b->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    b->last_blocks = blocks_run;
+    return MDAS_OK;
+}
+
+int mdas_batch_info(mdas_batch_handle b, double* wall_ms, int* pass_blocks, int* groups) {
+    if (!b) return mdas::set_error(MDAS_ERR_INVALID, "null handle");
+    if (wall_ms) *wall_ms = b->last_ms;
+    if (pass_blocks) *pass_blocks = b->last_blocks;
+    if (groups) *groups = (int)b->groups.size();
+    return MDAS_OK;
+}
+
+}  // extern "C"

@@ -69,6 +69,7 @@ struct DevGrid {
     unsigned char* rcnt;   // [cap_l]      reconstruction tiles per leaf
     DevCounters* cnt;
     PassStat* pstats;
+    int D;
     int cap_s, cap_l;
     int cand;
     int b, e;
@@ -96,6 +97,7 @@ struct Grid {
     unsigned* rs_hist = nullptr;  // radix histogram [256 * kSortBlocks]
     int tile_w = 8, tile_h = 8;
     long long last_n_keys = 0;
+    long long rrec_cap = 0;       // leaves the reconstruction records can hold
     int* d_scalar = nullptr;      // small device scratch
     // timing: per-kernel events when enabled
     bool timing = false;
@@ -151,9 +153,18 @@ hipError_t launch_arm(Grid& g, hipStream_t st);
 hipError_t launch_fill_i32(int* p, int v, long long n, hipStream_t st);
 hipError_t launch_fill_f32(float* p, float v, long long n, hipStream_t st);
 
+// Batched forms over a device array of n grids of dimension D (k_sampling.hip)
+hipError_t launch_initial_b(int D, const DevGrid* grids, int n, int blocks, const int* limits,
+                            hipStream_t st);
+hipError_t launch_pass_b(int D, const DevGrid* grids, int n, int blocks, hipStream_t st);
+
 // Integrand (k_integrand.hip)
 hipError_t launch_eval(Grid& g, const void* dparams, int first, int count, bool dev_range,
                        hipStream_t st);
+// Batched integrand over the new slice of every grid; params[i] is image i's
+// device parameter block.
+hipError_t launch_eval_b(const DevGrid* grids, const void* const* params, int n, int blocks,
+                         hipStream_t st);
 
 // Reconstruction (k_recon.hip)
 hipError_t launch_integrate_gather(Grid& g, float* pixels, int W, int H, hipStream_t st,

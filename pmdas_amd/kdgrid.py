@@ -50,6 +50,43 @@ class Scene:
             pass
 
 
+class Batch:
+    """Many grids advanced together (one launch per kernel and pass for all
+    images of a dimension; optionally replayed from a captured hipGraph)."""
+
+    def __init__(self, grids, scenes):
+        lib = _lib.load()
+        n = len(grids)
+        gh = (ctypes.c_void_p * n)(*[g.handle for g in grids])
+        sh = (ctypes.c_void_p * n)(*[s.handle for s in scenes])
+        h = ctypes.c_void_p()
+        check(lib.mdas_batch_create(gh, sh, n, ctypes.byref(h)))
+        self._h = h.value
+        self.grids, self.scenes = list(grids), list(scenes)
+
+    def run(self, targets, use_graph=True, stream=None):
+        t = (ctypes.c_longlong * len(targets))(*[int(x) for x in targets])
+        check(_lib.load().mdas_batch_run(self._h, t, stream.handle if stream else None,
+                                         1 if use_graph else 0))
+
+    def info(self):
+        ms, blocks, groups = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+        check(_lib.load().mdas_batch_info(self._h, ctypes.byref(ms), ctypes.byref(blocks),
+                                          ctypes.byref(groups)))
+        return dict(wall_ms=ms.value, pass_blocks=blocks.value, groups=groups.value)
+
+    def close(self):
+        if self._h:
+            _lib.load().mdas_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class KDGrid:
     """mdas::KDGrid<PointN<dim>> (kdgrid.h:42-51 constructor arguments)."""
 

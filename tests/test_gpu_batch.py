@@ -1,0 +1,81 @@
+"""Batched runs (one launch per kernel and pass for all images of a
+dimension, replayed from a captured hipGraph): every image must equal the
+same image run alone, bit for bit, and the oracle."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+# kind, dim, b, e, W, H, spp, alpha, cand, seed
+IMAGES = [
+    (3, 5, 1, 2, 64, 48, 12, 0.25, 4, 1),
+    (3, 5, 1, 3, 96, 64, 10, 0.25, 4, 2),
+    (3, 5, 0, 4, 40, 40, 16, 1 / 32, 4, 3),
+    (3, 5, 1, 2, 128, 72, 8, 0.25, 2, 4),
+    (2, 4, 1, 3, 80, 60, 12, 0.25, 4, 5),
+    (4, 6, 1, 1, 48, 32, 9, 1 / 64, 4, 6),
+]
+
+
+def _make(kdgrid, spec):
+    kind, dim, b, e, W, H, spp, alpha, cand, seed = spec
+    target = int(spp * W * H)
+    cap = int(target * 1.6) + (4 << (b * dim + 2 * e))
+    g = kdgrid.KDGrid(dim, cap, cand, b, e, alpha, float(W), float(H))
+    sc = kdgrid.Scene(kind, seed, dim, float(W), float(H))
+    return g, sc, target, (dim, cap, cand, b, e, alpha, float(W), float(H))
+
+
+@pytest.fixture(scope="module")
+def kd():
+    import pmdas_amd
+    from pmdas_amd import kdgrid
+    pmdas_amd.load()
+    return kdgrid
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_equals_single_runs(kd, use_graph):
+    made = [_make(kd, s) for s in IMAGES]
+    batch = kd.Batch([m[0] for m in made], [m[1] for m in made])
+    targets = [m[2] for m in made]
+    for replay in range(3):  # graph replays with eager work in between
+        print(f"batch replay {replay} graph={use_graph}", flush=True)
+        batch.run(targets, use_graph=use_graph)
+        states = [m[0].download_state() for m in made]
+        for (g, sc, target, args), st in zip(made, states):
+            solo = kd.KDGrid(*args)
+            solo.Run(sc, target)
+            ref = solo.download_state()
+            for k in st:
+                assert np.array_equal(st[k], ref[k]), (replay, k, args)
+            assert g.GetNumberOfSamples() >= target
+            solo.close()
+        # eager work on one of the grids between replays
+        made[0][0].Run(made[0][1], targets[0] // 2)
+    info = batch.info()
+    assert info["groups"] == 3 and info["pass_blocks"] > 0
+    batch.close()
+
+
+def test_batch_matches_oracle(kd):
+    made = [_make(kd, s) for s in IMAGES[:3]]
+    batch = kd.Batch([m[0] for m in made], [m[1] for m in made])
+    batch.run([m[2] for m in made], use_graph=True)
+    for g, sc, target, args in made:
+        o = pyoracle.OracleGrid(*args)
+        o.run(sc.params_bytes(), target)
+        st, ref = g.download_state(), o.state()
+        for k in st:
+            assert np.array_equal(st[k], ref[k]), k
+    batch.close()
+
+
+def test_batch_rejects_mismatched_scene(kd):
+    from pmdas_amd import MdasError
+    g, _, _, _ = _make(kd, IMAGES[0])
+    sc4 = kd.Scene(2, 0, 4, 64.0, 48.0)
+    with pytest.raises(MdasError):
+        kd.Batch([g], [sc4])
