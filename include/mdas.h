@@ -119,6 +119,14 @@ int mdas_eval_integrand(mdas_handle h, mdas_scene_handle s, int offset, int coun
 /* Integrand over the slice of the last adaptive pass, read on the device. */
 int mdas_eval_new_samples(mdas_handle h, mdas_scene_handle s, void* stream);
 
+/* The reference's plain (non-MDAS) sampler over the same scene
+ * (SDK/cuda/sampler.h:4-28, whitted.cu:59-83): per pixel, Halton points
+ * first_sample .. first_sample+spp-1 with TEA-seeded Cranley-Patterson
+ * offsets (seed tea4(pixel, frame)).  `out` is a device float4 image, or a
+ * double4 image evaluated in FP64 when fp64 != 0 (ground truth). */
+int mdas_render_qmc(mdas_scene_handle s, int width, int height, int spp, int first_sample,
+                    uint32_t frame, int fp64, void* out, void* stream);
+
 /* Whole sampling loop without per-pass host round trips: initial sampling,
  * integrand, then passes while N < min(target_samples, max_samples)
  * (optixMotionBlur.cpp:768-791).  The stopping rule is evaluated on the

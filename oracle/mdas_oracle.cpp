@@ -555,6 +555,59 @@ struct orc_grid {
     default: return -1;                             \
     }
 
+// cuda/random.h:74-89: radical inverse of i in base primes[dim]; the
+// reference divides f by the base each digit and takes floor(j / base).
+template <typename real>
+real halton(int dim, int i) {
+    static const int primes[6] = {2, 3, 5, 7, 11, 13};
+    const int base = primes[dim];
+    real r = 0, f = (real)1 / (real)base;
+    for (int j = i; j > 0; j /= base) {
+        r += f * (real)(j % base);
+        f /= (real)base;
+    }
+    return r;
+}
+
+// cuda/sampler.h:4-28 + whitted.cu:59-69: seed = tea4(pixel, frame), six
+// offsets from rnd(seed), sample s of dim d = halton(d, s) + offset[d], minus
+// 1 when above 1; dims 0/1 are the jitter inside pixel (px, py), mapped to
+// the MDAS image coordinates [0, scale_x) x [0, scale_y).  Pixel value = sum
+// of the spp values in sample order times 1/spp.
+template <typename real>
+void qmc_image(const mscene::Params& P, int W, int H, int spp, int first, uint32_t frame,
+               real* out) {
+    const int dim = P.dim;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads())
+    for (long long pix = 0; pix < (long long)W * H; ++pix) {
+        const int px = (int)(pix % W), py = (int)(pix / W);
+        uint32_t seed = tea4((uint32_t)pix, frame);
+        real off[6];
+        for (int d = 0; d < 6; ++d) off[d] = (real)rnd(seed);
+        real acc[3] = {0, 0, 0};
+        for (int s = first; s < first + spp; ++s) {
+            real x[6];
+            for (int d = 0; d < dim; ++d) {
+                real v = halton<real>(d, s) + off[d];
+                if (v > (real)1) v -= (real)1;
+                x[d] = v;
+            }
+            x[0] = ((real)px + x[0]) * (real)P.scale_x / (real)W;
+            x[1] = ((real)py + x[1]) * (real)P.scale_y / (real)H;
+            const mscene::V3<real> v = mscene::eval<real>(P, P.obj, x);
+            acc[0] += v.x;
+            acc[1] += v.y;
+            acc[2] += v.z;
+        }
+        const real inv = (real)1 / (real)spp;
+        real* o = out + pix * 4;
+        o[0] = acc[0] * inv;
+        o[1] = acc[1] * inv;
+        o[2] = acc[2] * inv;
+        o[3] = 0;
+    }
+}
+
 extern "C" {
 
 orc_grid* orc_create(int dim, long long max_samples, int candidates_num, int bits_per_dim,
@@ -643,6 +696,17 @@ void orc_eval_scene_f64(const void* params, const double* coords, int dim, long 
         values[3 * k + 1] = v.y;
         values[3 * k + 2] = v.z;
     }
+}
+
+float orc_halton(int dim, int i) { return halton<float>(dim, i); }
+
+void orc_qmc_image(const void* params, int w, int h, int spp, int first, uint32_t frame, int fp64,
+                   void* out) {
+    const mscene::Params& P = *(const mscene::Params*)params;
+    if (fp64)
+        qmc_image<double>(P, w, h, spp, first, frame, (double*)out);
+    else
+        qmc_image<float>(P, w, h, spp, first, frame, (float*)out);
 }
 
 void orc_eval_range(orc_grid* h, const void* params, long long first, long long count) {

@@ -51,6 +51,11 @@ void orc_eval_scene(const void* scene_params, const float* coords, int dim, long
 void orc_eval_range(orc_grid* g, const void* scene_params, long long first, long long count);
 void orc_eval_scene_f64(const void* scene_params, const double* coords, int dim,
                         long long count, double* values);
+/* The plain QMC sampler's image (cuda/sampler.h:4-28, random.h:74-89,
+ * whitted.cu:59-69): float4 pixels (fp64 = 0) or double4 (fp64 = 1). */
+void orc_qmc_image(const void* scene_params, int w, int h, int spp, int first_sample,
+                   uint32_t frame, int fp64, void* out);
+float orc_halton(int dim, int i);
 
 /* Reconstruction keys: (morton2(tile_x, tile_y) << 32) | leaf, sorted. */
 long long orc_recon_keys(orc_grid* g, int w, int h, int tile_w, int tile_h, uint64_t* out,

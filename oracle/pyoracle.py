@@ -34,6 +34,8 @@ _SIGS = {
     "orc_make_scene": (_LL, [_I, _U, _I, _F, _F, _P, _LL]),
     "orc_eval_scene": (None, [_P, _P, _I, _LL, _LL, _P]),
     "orc_eval_scene_f64": (None, [_P, _P, _I, _LL, _P]),
+    "orc_qmc_image": (None, [_P, _I, _I, _I, _I, _U, _I, _P]),
+    "orc_halton": (_F, [_I, _I]),
     "orc_eval_range": (None, [_P, _P, _LL, _LL]),
     "orc_recon_keys": (_LL, [_P, _I, _I, _I, _I, _P, _LL]),
     "orc_num_samples": (_LL, [_P]),
@@ -218,6 +220,17 @@ def eval_scene_f64(params, coords, dim):
     out = np.empty((len(c), 3), np.float64)
     load().orc_eval_scene_f64(params, vp(c), dim, len(c), vp(out))
     return out
+
+
+def qmc_image(params, w, h, spp, first_sample=0, frame=0, fp64=False):
+    """The plain QMC sampler's (H, W, 4) image (float32, or float64 if fp64)."""
+    out = np.empty((h, w, 4), np.float64 if fp64 else np.float32)
+    load().orc_qmc_image(params, w, h, spp, first_sample, frame, 1 if fp64 else 0, vp(out))
+    return out
+
+
+def halton(dim, i):
+    return load().orc_halton(dim, i)
 
 
 def powf(x, a):

@@ -117,6 +117,7 @@ SIGNATURES = {
     "mdas_scene_params": (_LL, [_H, _P, _LL]),
     "mdas_eval_integrand": (_I, [_H, _H, _I, _I, _P]),
     "mdas_eval_new_samples": (_I, [_H, _H, _P]),
+    "mdas_render_qmc": (_I, [_H, _I, _I, _I, _I, ctypes.c_uint32, _I, _P, _P]),
     "mdas_run": (_I, [_H, _H, _LL, _P]),
     "mdas_get_passes": (_I, [_H]),
     "mdas_set_timing": (_I, [_H, _I]),

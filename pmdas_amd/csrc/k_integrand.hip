@@ -52,6 +52,79 @@ __global__ __launch_bounds__(256) void k_eval_b(const DevGrid* grids, const void
               blockIdx.x, gridDim.x);
 }
 
+// ------------------------------------------------- QMC and ground truth --
+// The reference's non-MDAS sampler (SDK/cuda/sampler.h:4-28, random.h:75-89,
+// whitted.cu:59-83): per pixel a TEA-seeded set of 6 Cranley-Patterson
+// offsets, sample s takes the Halton point s (bases 2, 3, 5, 7, 11, 13)
+// shifted by the offsets and wrapped when above 1; dims 0 and 1 jitter inside
+// the pixel.
+// `real` = float gives the MC/QMC baseline images, double the FP64
+// high-sample-count ground truth.
+template <typename real>
+__device__ __forceinline__ real halton_r(int dim, int i) {
+    const int primes[6] = {2, 3, 5, 7, 11, 13};
+    const int base = primes[dim];
+    real r = 0, f = (real)1 / (real)base;
+    int j = i;
+    while (j > 0) {
+        r += f * (real)(j % base);
+        j /= base;
+        f /= (real)base;
+    }
+    return r;
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_qmc_image(const mscene::Params* __restrict__ P, int W,
+                                                   int H, int spp, int first_sample,
+                                                   unsigned frame, real* out) {
+    __shared__ float obj[mscene::MAX_OBJ * mscene::OBJ_STRIDE];
+    const int nobj = P->n_obj;
+    for (int k = threadIdx.x; k < nobj * mscene::OBJ_STRIDE; k += 256) obj[k] = P->obj[k];
+    __syncthreads();
+    const int dim = P->dim;
+    for (int pix = blockIdx.x * 256 + threadIdx.x; pix < W * H; pix += gridDim.x * 256) {
+        const int px = pix % W, py = pix / W;
+        unsigned seed = tea4((unsigned)pix, frame);
+        real off[6];
+        for (int d = 0; d < 6; ++d) off[d] = (real)rnd(seed);
+        real ax = 0, ay = 0, az = 0;
+        for (int s = first_sample; s < first_sample + spp; ++s) {
+            real x[6];
+            for (int d = 0; d < dim; ++d) {
+                real v = halton_r<real>(d, s) + off[d];
+                if (v > (real)1) v -= (real)1;
+                x[d] = v;
+            }
+            x[0] = ((real)px + x[0]) * (real)P->scale_x / (real)W;
+            x[1] = ((real)py + x[1]) * (real)P->scale_y / (real)H;
+            const mscene::V3<real> v = mscene::eval<real>(*P, obj, x);
+            ax += v.x;
+            ay += v.y;
+            az += v.z;
+        }
+        real* o = out + (size_t)pix * 4;
+        const real inv = (real)1 / (real)spp;
+        o[0] = ax * inv;
+        o[1] = ay * inv;
+        o[2] = az * inv;
+        o[3] = 0;
+    }
+}
+
+hipError_t launch_qmc_image(const void* dparams, int W, int H, int spp, int first_sample,
+                            unsigned frame, bool fp64, void* out, hipStream_t st) {
+    long long blocks = ((long long)W * H + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (fp64)
+        hipLaunchKernelGGL(k_qmc_image<double>, dim3((unsigned)blocks), dim3(256), 0, st,
+                           (const mscene::Params*)dparams, W, H, spp, first_sample, frame, (double*)out);
+    else
+        hipLaunchKernelGGL(k_qmc_image<float>, dim3((unsigned)blocks), dim3(256), 0, st,
+                           (const mscene::Params*)dparams, W, H, spp, first_sample, frame, (float*)out);
+    return hipGetLastError();
+}
+
 hipError_t launch_eval(Grid& g, const void* dparams, int first, int count, bool dev_range,
                        hipStream_t st) {
     long long n = dev_range ? (long long)g.cap_s : (long long)count;

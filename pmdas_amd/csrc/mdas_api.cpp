@@ -609,6 +609,15 @@ int mdas_eval_new_samples(mdas_handle h, mdas_scene_handle s, void* stream) {
     return MDAS_OK;
 }
 
+int mdas_render_qmc(mdas_scene_handle s, int w, int hgt, int spp, int first_sample, uint32_t frame,
+                    int fp64, void* out, void* stream) {
+    if (!s || !out) return fail(MDAS_ERR_INVALID, "null argument");
+    if (w <= 0 || hgt <= 0 || spp <= 0 || first_sample < 0)
+        return fail(MDAS_ERR_INVALID, "image size and spp must be positive");
+    HIP_TRY(mdas::launch_qmc_image(s->dev, w, hgt, spp, first_sample, frame, fp64 != 0, out, S(stream)));
+    return MDAS_OK;
+}
+
 // ---------------------------------------------------------------- the loop --
 int mdas_set_timing(mdas_handle h, int enable) {
     if (!h) return fail(MDAS_ERR_INVALID, "null handle");

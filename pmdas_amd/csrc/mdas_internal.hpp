@@ -165,6 +165,9 @@ hipError_t launch_eval(Grid& g, const void* dparams, int first, int count, bool 
 // device parameter block.
 hipError_t launch_eval_b(const DevGrid* grids, const void* const* params, int n, int blocks,
                          hipStream_t st);
+// Per-pixel QMC image (float) or FP64 ground truth (double), float4/double4 out.
+hipError_t launch_qmc_image(const void* dparams, int W, int H, int spp, int first_sample,
+                            unsigned frame, bool fp64, void* out, hipStream_t st);
 
 // Reconstruction (k_recon.hip)
 hipError_t launch_integrate_gather(Grid& g, float* pixels, int W, int H, hipStream_t st,

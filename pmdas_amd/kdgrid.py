@@ -31,6 +31,18 @@ class Scene:
     def handle(self):
         return self._h
 
+    def render_qmc(self, width, height, spp, first_sample=0, frame=0, fp64=False):
+        """The plain QMC sampler's image (H, W, 4); FP64 evaluation if fp64."""
+        dt = np.float64 if fp64 else np.float32
+        buf = hip.DeviceBuffer(width * height * 4 * np.dtype(dt).itemsize)
+        try:
+            check(_lib.load().mdas_render_qmc(self._h, width, height, spp, first_sample, frame,
+                                              1 if fp64 else 0, buf.ptr, None))
+            hip.synchronize()
+            return buf.download(dt, width * height * 4).reshape(height, width, 4)
+        finally:
+            buf.free()
+
     def params_bytes(self):
         lib = _lib.load()
         n = lib.mdas_scene_params(self._h, None, 0)

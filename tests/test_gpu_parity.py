@@ -268,3 +268,15 @@ def test_full_size_4d_dof_properties(mdas):
     px = g.integrate_image(W, H)
     assert np.isfinite(px).all() and (px[..., :3] >= 0).all()
     assert (px[..., 3] == 0).all()
+
+
+@pytest.mark.parametrize("kind,dim,fp64", [(5, 3, False), (3, 5, False), (4, 6, False),
+                                           (2, 4, True), (3, 5, True)])
+def test_render_qmc_bit_exact(mdas, kind, dim, fp64):
+    _say(f"render_qmc kind={kind} dim={dim} fp64={fp64}")
+    W, H, sx, sy = 40, 24, 40.0, 24.0
+    sc = mdas.Scene(kind, 11, dim, sx, sy)
+    got = sc.render_qmc(W, H, 12, first_sample=3, frame=2, fp64=fp64)
+    want = pyoracle.qmc_image(sc.params_bytes(), W, H, 12, 3, 2, fp64=fp64)
+    u = np.uint64 if fp64 else np.uint32
+    assert np.array_equal(got.view(u), want.view(u))
