@@ -216,6 +216,66 @@ const char* mdas_env_last_error(void);
 /* KDGrid arguments of the app (optixMotionBlur.cpp:336-350). */
 int mdas_env_config(const mdas_env* env, long long max_samples, mdas_config* cfg);
 
+/* ------------------------------------------------------------------------
+ * Learned proposal (no reference counterpart: a build addition beside the
+ * plain method).  A 16 -> 64 -> 64 -> 1 MLP with GELU (tanh form), BF16
+ * weights and layer inputs on MFMA, FP32 accumulation, predicts from 16 leaf
+ * features the log2 of the relative error reduction a split achieves,
+ * r = max(e_parent - e_left - e_right, 2^-10 e_parent) / e_parent, measured
+ * by the plain error estimate.  With scoring on, selection uses
+ * e * 2^clamp(prediction, -10, 0) in place of the error e in
+ * pow(score / max score, alpha) >= u; the plain error remains the fallback
+ * (no network or a non-finite output) and is still computed for every leaf.
+ *
+ * Parameter blob (MDAS_MLP_FLOATS floats): W1[64][16], b1[64], W2[64][64],
+ * b2[64], w3[64], b3 -- the MDAS_MLP_PARAMS trained values -- then the input
+ * normalisation mu[16] and 1/sigma[16] (x' = (x - mu) / sigma).
+ * Features (MDAS_FEATURES floats per leaf): log2 extent per dimension over
+ * the domain extent (6 values, 0 above D), sample count, mean, variance,
+ * min and max luminance (0.2126 R + 0.7152 G + 0.0722 B), depth (splits
+ * since the initial grid), log2 error, log2 error / volume, log2 volume, D.
+ * All row and label pointers are device memory. */
+typedef struct mdas_mlp* mdas_mlp_handle;
+#define MDAS_MLP_FLOATS 5345
+#define MDAS_MLP_PARAMS 5313
+#define MDAS_FEATURES 16
+#define MDAS_LEARN_COLLECT 1 /* write (parent features, label) rows of every split */
+#define MDAS_LEARN_SCORE 2   /* select by the network's score */
+
+/* Seeded default initialisation: U(-1/sqrt(fan_in), 1/sqrt(fan_in)). */
+int mdas_mlp_create(uint32_t seed, mdas_mlp_handle* out);
+int mdas_mlp_destroy(mdas_mlp_handle m);
+/* Replace / read the blob; setting it also resets the Adam state. */
+int mdas_mlp_set_params(mdas_mlp_handle m, const float* host, int n);
+int mdas_mlp_get_params(mdas_mlp_handle m, float* host, int n);
+/* mu and 1/sigma of the n rows (FP64 sums). */
+int mdas_mlp_fit_norm(mdas_mlp_handle m, const float* rows, long long n, void* stream);
+/* out[i] = network(rows[i]) for n rows of MDAS_FEATURES raw features. */
+int mdas_mlp_forward(mdas_mlp_handle m, const float* rows, long long n, float* out, void* stream);
+/* Gradient of mean((out - target)^2) over the n rows, summed over rows in a
+ * fixed order; grad_host receives MDAS_MLP_PARAMS floats, loss_host the
+ * loss.  Synchronises. */
+int mdas_mlp_grad(mdas_mlp_handle m, const float* rows, const float* targets, long long n,
+                  float* grad_host, float* loss_host, void* stream);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) over `epochs` passes of the rows in
+ * minibatches of `batch`, each epoch in a seeded permutation
+ * q -> (a q + c) mod n; epoch_loss_host (optional, `epochs` floats) gets the
+ * mean loss seen during each epoch.  Synchronises. */
+int mdas_mlp_train(mdas_mlp_handle m, const float* rows, const float* targets, long long n,
+                   int epochs, int batch, float lr, uint32_t seed, float* epoch_loss_host,
+                   void* stream);
+/* Learned mode of a grid for mdas_run (0 = the plain method).  COLLECT
+ * writes, for the j-th split leaf of each pass, its features and label to
+ * rows / labels at index (splits of earlier passes) + j while that index is
+ * below cap_rows; SCORE needs `m`, which must outlive the grid's runs.
+ * Resets the row counter. */
+int mdas_set_learned(mdas_handle h, int mode, mdas_mlp_handle m, float* rows, float* labels,
+                     long long cap_rows);
+/* Rows offered since mdas_set_learned (those beyond cap_rows were dropped). */
+int mdas_get_learned_rows(mdas_handle h, long long* offered);
+/* Features and plain errors of leaves [0, n) after a learned-mode run. */
+int mdas_get_leaf_features(mdas_handle h, float* feat_host, float* perr_host, long long n);
+
 const char* mdas_last_error(void);
 int mdas_device_synchronize(void);
 

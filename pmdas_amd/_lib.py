@@ -23,6 +23,12 @@ SCENE_SPHERES_MBDOF = 3
 SCENE_AO = 4
 SCENE_RAMP = 5
 
+MLP_FLOATS = 5345
+MLP_PARAMS = 5313
+FEATURES = 16
+LEARN_COLLECT = 1
+LEARN_SCORE = 2
+
 
 class MdasError(RuntimeError):
     def __init__(self, code, msg):
@@ -139,6 +145,17 @@ SIGNATURES = {
     "mdas_env_read": (_I, [ctypes.c_char_p, ctypes.POINTER(MdasEnv)]),
     "mdas_env_last_error": (ctypes.c_char_p, []),
     "mdas_env_config": (_I, [ctypes.POINTER(MdasEnv), _LL, ctypes.POINTER(MdasConfig)]),
+    "mdas_mlp_create": (_I, [ctypes.c_uint32, ctypes.POINTER(_H)]),
+    "mdas_mlp_destroy": (_I, [_H]),
+    "mdas_mlp_set_params": (_I, [_H, _P, _I]),
+    "mdas_mlp_get_params": (_I, [_H, _P, _I]),
+    "mdas_mlp_fit_norm": (_I, [_H, _P, _LL, _P]),
+    "mdas_mlp_forward": (_I, [_H, _P, _LL, _P, _P]),
+    "mdas_mlp_grad": (_I, [_H, _P, _P, _LL, _P, ctypes.POINTER(_F), _P]),
+    "mdas_mlp_train": (_I, [_H, _P, _P, _LL, _I, _I, _F, ctypes.c_uint32, _P, _P]),
+    "mdas_set_learned": (_I, [_H, _I, _H, _P, _P, _LL]),
+    "mdas_get_learned_rows": (_I, [_H, ctypes.POINTER(_LL)]),
+    "mdas_get_leaf_features": (_I, [_H, _P, _P, _LL]),
     "mdas_last_error": (ctypes.c_char_p, []),
     "mdas_device_synchronize": (_I, []),
 }

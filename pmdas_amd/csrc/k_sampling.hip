@@ -130,57 +130,6 @@ __global__ void k_arm(DevCounters* c) {
 }
 
 // ------------------------------------------------------------------ errors --
-// kdgrid.cu:237-266: relative-L1 error of a leaf's samples times its volume.
-// *nvals returns the number of sample values read.
-template <int D>
-__device__ __forceinline__ float leaf_error(const DevGrid& a, int i, int* nvals) {
-    int4 id;
-    float mn[D], mx[D];
-    load_leaf<D>(a, i, id, mn, mx);
-    const int ids[4] = {id.x, id.y, id.z, id.w};
-    float vx[4] = {0.0f, 0.0f, 0.0f, 0.0f}, vy[4] = {0.0f, 0.0f, 0.0f, 0.0f},
-          vz[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    int n = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        if (ids[q] >= 0) {
-            const float* v = a.values + (size_t)ids[q] * 3;
-            vx[q] = v[0];
-            vy[q] = v[1];
-            vz[q] = v[2];
-            sx += vx[q];
-            sy += vy[q];
-            sz += vz[q];
-            ++n;
-        }
-    }
-    // float3 /= float is a reciprocal multiply (sutil/vec_math.h:492-496)
-    const float inv = 1.0f / (float)n;
-    const float ax = sx * inv, ay = sy * inv, az = sz * inv;
-    float dx = 0.0f, dy = 0.0f, dz = 0.0f;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        if (q < n) {
-            dx += fabsf(vx[q] - ax);
-            dy += fabsf(vy[q] - ay);
-            dz += fabsf(vz[q] - az);
-        }
-    }
-    float vol = 1.0f;
-#pragma unroll
-    for (int d = 0; d < D; ++d) vol *= (mx[d] - mn[d]);
-    float e = 0.0f;
-    if (ax != 0.0f) e += dx / ax;
-    if (ay != 0.0f) e += dy / ay;
-    if (az != 0.0f) e += dz / az;
-    e /= (float)n;
-    e += 1.0e-5f;
-    e *= vol;
-    *nvals = n;
-    return e;
-}
-
 // Workgroup reduction of (max, stale count, values read) into epart[block].
 __device__ __forceinline__ void errors_epilogue(const DevGrid& a, float m, int n_stale, int n_vals,
                                                 float* wmax, int* wst, int blk) {

@@ -46,7 +46,7 @@ void free_grid(Grid& g) {
     void* ptrs[] = {d.coords,   d.values, d.leaf,     d.err,  d.seed,       d.flags,
                     d.split_list, d.part, d.part_off, d.epart, d.rrec,      d.cnt,
                     d.pstats,   g.keys,   g.keys_alt, g.tile_start, g.tile_end, g.rs_hist,
-                    g.d_scalar};
+                    g.d_scalar, g.learn.feat, g.learn.perr, g.learn.rows};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : g.evpool) (void)hipEventDestroy(e);
@@ -56,6 +56,8 @@ void free_grid(Grid& g) {
     g.tile_start = g.tile_end = nullptr;
     g.rs_hist = nullptr;
     g.d_scalar = nullptr;
+    g.learn = mdas::LearnDev{};
+    g.learn_mode = 0;
 }
 
 int sync_counters(Grid& g, hipStream_t st) {
@@ -651,7 +653,8 @@ int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream)
         for (int b = 0; b < B; ++b) {
             mdas::kbeg(g, st, "k_errors");
             HIP_TRY(mdas::launch_pass_begin(g, st));
-            HIP_TRY(mdas::launch_errors_stale(g, st));
+            HIP_TRY(g.learn_mode ? mdas::launch_errors_learned(g, st)
+                                 : mdas::launch_errors_stale(g, st));
             mdas::kend(g, st);
             mdas::kbeg(g, st, "k_select");
             HIP_TRY(mdas::launch_select(g, st));

@@ -76,6 +76,32 @@ struct DevGrid {
     float alpha, sX, sY;
 };
 
+// One MLP of the learned proposal (16 -> 64 -> 64 -> 1).  p is the FP32
+// parameter blob (layout in mlp_device.hpp); the others are the bf16 forms
+// the forward reads (W1 packed [64][32] with zero k >= 16, W2 [64][64]) and
+// the bf16-rounded W2 and w3 as floats.
+struct MlpDev {
+    const float* p;
+    const unsigned short* w1p;
+    const unsigned short* w2p;
+    const float* w2r;
+    const float* w3r;
+};
+
+static const int kFeat = 16;  // features per leaf
+
+// Learned-proposal state of one grid; unused by the plain method.
+struct LearnDev {
+    float* feat;         // [cap_l][16] features of each leaf at its last error estimate
+    float* perr;         // [cap_l] plain error of each leaf (err holds the score)
+    float* X;            // [cap_rows][16] collected rows: features of split leaves
+    float* Y;            // [cap_rows] their labels; X == null: no collection
+    long long* rows;     // rows offered so far (index >= cap_rows: dropped)
+    long long cap_rows;
+    MlpDev mlp;          // mlp.p == null: the score is the plain error
+    float ylo, yhi;      // clamp of the predicted log2 reduction ratio
+};
+
 struct Grid {
     int D = 0;
     int cand = 4, b = 0, e = 0;
@@ -116,6 +142,9 @@ struct Grid {
     std::vector<PassStat> pstats_host;
     long long recon_pixels = 0;
     int recon_sort_passes = 0;
+    // learned proposal (mode bits: 1 collect rows, 2 score with the MLP)
+    int learn_mode = 0;
+    LearnDev learn{};
 };
 
 // Kernel timing brackets: no-ops unless g.timing.
@@ -174,6 +203,28 @@ hipError_t launch_integrate_gather(Grid& g, float* pixels, int W, int H, hipStre
                                    hipEvent_t ev_sort0, hipEvent_t ev_sort1);
 hipError_t launch_integrate_scatter(Grid& g, float* pixels, int W, int H, hipStream_t st);
 hipError_t launch_bytes(const float* pixels, unsigned char* out, long long n, hipStream_t st);
+
+// learned proposal (k_learned.hip)
+hipError_t launch_errors_learned(Grid& g, hipStream_t st);
+hipError_t launch_mlp_pack(float* p, unsigned short* w1p, unsigned short* w2p, float* w2r,
+                           float* w3r, hipStream_t st);
+hipError_t launch_mlp_forward(const MlpDev& M, const float* X, long long n, float* Y,
+                              hipStream_t st);
+// Per-wave partial gradients of mean((y - t)^2) over rows perm(q),
+// q in [start, start + count), perm(q) = (pa * q + pc) mod n_rows.  Returns
+// the number of wave partials written to gpart / lpart.
+int launch_mlp_grad(const MlpDev& M, const float* X, const float* T, long long n_rows,
+                    long long start, int count, unsigned long long pa, unsigned long long pc,
+                    float* gpart, float* lpart, hipStream_t st, hipError_t* err);
+hipError_t launch_mlp_sum(const float* gpart, const float* lpart, int nparts, float* gsum,
+                          float* loss, hipStream_t st);
+hipError_t launch_mlp_adam(float* p, float* m, float* v, const float* gpart, const float* lpart,
+                           int nparts, float lr, float bc1, float bc2, float* loss_acc,
+                           unsigned short* w1p, unsigned short* w2p, float* w2r, float* w3r,
+                           hipStream_t st);
+hipError_t launch_feat_stats(const float* X, long long n, double* part, int nblocks,
+                             hipStream_t st);
+static const int kMlpGradBlocks = 256;  // workgroups of the gradient kernel (4 waves each)
 hipError_t recon_prepare(Grid& g, int W, int H, hipStream_t st);
 
 }  // namespace mdas

@@ -4,6 +4,8 @@
 #include "mdas_device.hpp"
 #include "mdas_internal.hpp"
 
+#pragma clang fp contract(off)
+
 namespace mdas {
 
 template <int D>
@@ -78,6 +80,76 @@ __device__ __forceinline__ void store_leaf(const DevGrid& a, int i, int4 id, con
 #pragma unroll
     for (int k = 1; k < Q; ++k)
         p[k] = make_float4(b[4 * (k - 1)], b[4 * (k - 1) + 1], b[4 * (k - 1) + 2], b[4 * (k - 1) + 3]);
+}
+
+// kdgrid.cu:237-266: relative-L1 error of a leaf's samples times its volume.
+// Also returns the leaf's sample values by slot, its box and volume for
+// callers that derive more statistics from the same reads.  Occupied slots
+// are always 0..n-1: slots fill in order and a split writes both children
+// from slot 0.
+struct LeafSamples {
+    float vx[4], vy[4], vz[4];
+    int n;
+    float vol;
+};
+
+template <int D>
+__device__ __forceinline__ float leaf_error_s(const DevGrid& a, int i, LeafSamples& ls, float* mn,
+                                              float* mx) {
+    int4 id;
+    load_leaf<D>(a, i, id, mn, mx);
+    const int ids[4] = {id.x, id.y, id.z, id.w};
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    int n = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        ls.vx[q] = ls.vy[q] = ls.vz[q] = 0.0f;
+        if (ids[q] >= 0) {
+            const float* v = a.values + (size_t)ids[q] * 3;
+            ls.vx[q] = v[0];
+            ls.vy[q] = v[1];
+            ls.vz[q] = v[2];
+            sx += ls.vx[q];
+            sy += ls.vy[q];
+            sz += ls.vz[q];
+            ++n;
+        }
+    }
+    // float3 /= float is a reciprocal multiply (sutil/vec_math.h:492-496)
+    const float inv = 1.0f / (float)n;
+    const float ax = sx * inv, ay = sy * inv, az = sz * inv;
+    float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (q < n) {
+            dx += fabsf(ls.vx[q] - ax);
+            dy += fabsf(ls.vy[q] - ay);
+            dz += fabsf(ls.vz[q] - az);
+        }
+    }
+    float vol = 1.0f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) vol *= (mx[d] - mn[d]);
+    float e = 0.0f;
+    if (ax != 0.0f) e += dx / ax;
+    if (ay != 0.0f) e += dy / ay;
+    if (az != 0.0f) e += dz / az;
+    e /= (float)n;
+    e += 1.0e-5f;
+    e *= vol;
+    ls.n = n;
+    ls.vol = vol;
+    return e;
+}
+
+// *nvals returns the number of sample values read.
+template <int D>
+__device__ __forceinline__ float leaf_error(const DevGrid& a, int i, int* nvals) {
+    LeafSamples ls;
+    float mn[D], mx[D];
+    const float e = leaf_error_s<D>(a, i, ls, mn, mx);
+    *nvals = ls.n;
+    return e;
 }
 
 }  // namespace mdas

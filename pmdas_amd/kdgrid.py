@@ -299,9 +299,109 @@ class KDGrid:
         finally:
             buf.free()
 
+    # ------------------------------------------------ learned proposal --
+    def set_learned(self, mode, mlp=None, rows=None, labels=None, cap_rows=0):
+        """mode: 0 plain, LEARN_COLLECT | LEARN_SCORE bits; rows / labels are
+        DeviceBuffers (or device pointers) for collection."""
+        def ptr(b):
+            return None if b is None else (b.ptr if hasattr(b, "ptr") else b)
+        check(_lib.load().mdas_set_learned(self._h, mode, None if mlp is None else mlp.handle,
+                                           ptr(rows), ptr(labels), cap_rows))
+
+    def learned_rows(self):
+        n = ctypes.c_longlong(0)
+        check(_lib.load().mdas_get_learned_rows(self._h, ctypes.byref(n)))
+        return n.value
+
+    def leaf_features(self):
+        """(features (L, 16), plain errors (L,)) after a learned-mode run."""
+        L = self.GetNumberOfNodes()
+        f = np.empty((L, _lib.FEATURES), np.float32)
+        e = np.empty(L, np.float32)
+        check(_lib.load().mdas_get_leaf_features(self._h, f.ctypes.data_as(ctypes.c_void_p),
+                                                 e.ctypes.data_as(ctypes.c_void_p), L))
+        return f, e
+
     def close(self):
         if self._h:
             _lib.load().mdas_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Mlp:
+    """The learned proposal's network (16 -> 64 -> 64 -> 1, GELU, BF16 on
+    MFMA, FP32 accumulation) with its training on the GPU."""
+
+    def __init__(self, seed=0):
+        self._h = ctypes.c_void_p()
+        check(_lib.load().mdas_mlp_create(seed, ctypes.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def params(self):
+        out = np.empty(_lib.MLP_FLOATS, np.float32)
+        check(_lib.load().mdas_mlp_get_params(self._h, out.ctypes.data_as(ctypes.c_void_p),
+                                              _lib.MLP_FLOATS))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        check(_lib.load().mdas_mlp_set_params(self._h, p.ctypes.data_as(ctypes.c_void_p), p.size))
+
+    def fit_norm(self, rows_dev, n):
+        check(_lib.load().mdas_mlp_fit_norm(self._h, rows_dev.ptr, n, None))
+
+    def forward(self, X):
+        """Host rows (n, 16) -> host outputs (n,), through device buffers."""
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, _lib.FEATURES)
+        n = len(X)
+        bx, by = hip.DeviceBuffer(X.nbytes), hip.DeviceBuffer(4 * n)
+        try:
+            bx.upload(X)
+            check(_lib.load().mdas_mlp_forward(self._h, bx.ptr, n, by.ptr, None))
+            hip.synchronize()
+            return by.download(np.float32, n)
+        finally:
+            bx.free()
+            by.free()
+
+    def grad(self, X, T):
+        """(gradient (MLP_PARAMS,), loss) of mean((y - T)^2) over host rows."""
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, _lib.FEATURES)
+        T = np.ascontiguousarray(T, np.float32)
+        n = len(X)
+        bx, bt = hip.DeviceBuffer(X.nbytes), hip.DeviceBuffer(T.nbytes)
+        g = np.empty(_lib.MLP_PARAMS, np.float32)
+        loss = ctypes.c_float(0.0)
+        try:
+            bx.upload(X)
+            bt.upload(T)
+            check(_lib.load().mdas_mlp_grad(self._h, bx.ptr, bt.ptr, n,
+                                            g.ctypes.data_as(ctypes.c_void_p), ctypes.byref(loss),
+                                            None))
+            return g, loss.value
+        finally:
+            bx.free()
+            bt.free()
+
+    def train(self, rows_dev, labels_dev, n, epochs, batch=65536, lr=1e-3, seed=0):
+        """Adam on device rows; returns the mean loss of each epoch."""
+        losses = np.zeros(epochs, np.float32)
+        check(_lib.load().mdas_mlp_train(self._h, rows_dev.ptr, labels_dev.ptr, n, epochs, batch,
+                                         lr, seed, losses.ctypes.data_as(ctypes.c_void_p), None))
+        return losses
+
+    def close(self):
+        if self._h:
+            _lib.load().mdas_mlp_destroy(self._h)
             self._h = None
 
     def __del__(self):
