@@ -47,9 +47,15 @@ __device__ __forceinline__ void initial_body(const DevGrid& a, int L0, int blk, 
             a.flags[i] = 4;
         }
         __syncthreads();
+        // leaf i's samples are 4i..4i+3, so the per-leaf point copies of the
+        // workgroup's leaves are the same contiguous range
         const int nl = min(256, L0 - base);
         float* out = a.coords + (size_t)base * 4 * D;
-        for (int k = threadIdx.x; k < nl * 4 * D; k += 256) out[k] = stage[k];
+        float* outp = a.pts + (size_t)base * 4 * D;
+        for (int k = threadIdx.x; k < nl * 4 * D; k += 256) {
+            out[k] = stage[k];
+            outp[k] = stage[k];
+        }
         __syncthreads();
     }
 }
@@ -414,13 +420,15 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
 #pragma unroll
     for (int d = 0; d < D; ++d) diag[d] = mx[d] - mn[d];
     const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
+    // the leaf's points from its contiguous copy (slot order) instead of
+    // gathers at the sample indices
     float p[4][D];
+    float* lp = a.pts + (size_t)i * 4 * D;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         if (ids[q] >= 0) {
-            const float* c = a.coords + (size_t)ids[q] * D;
 #pragma unroll
-            for (int d = 0; d < D; ++d) p[q][d] = c[d];
+            for (int d = 0; d < D; ++d) p[q][d] = lp[q * D + d];
         } else {
 #pragma unroll
             for (int d = 0; d < D; ++d) p[q][d] = 0.0f;
@@ -471,6 +479,8 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
         else if (cnt == 2u) n4.z = k;
         else n4.w = k;
         store_slots(a, i, Q, n4);
+#pragma unroll
+        for (int d = 0; d < D; ++d) lp[cnt * D + d] = best[d];
         a.flags[i] = (unsigned char)(cnt + 1u);
     } else {
         // kdgrid.cu:399-471: split.  LargestAxis scans from the last axis down
@@ -485,6 +495,7 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
             }
         float key[5];
         int v[5] = {ids[0], ids[1], ids[2], ids[3], k};
+        int perm[5] = {0, 1, 2, 3, 4};  // which of p[0..3], best sits at each position
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             float kv = p[q][0];
@@ -512,6 +523,9 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
                     int tv = v[q];
                     v[q] = v[q + 1];
                     v[q + 1] = tv;
+                    tv = perm[q];
+                    perm[q] = perm[q + 1];
+                    perm[q + 1] = tv;
                 }
             }
         }
@@ -525,6 +539,23 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
         }
         store_leaf<D>(a, i, make_int4(v[0], v[1], E, E), mn, cmx);
         store_leaf<D>(a, m, make_int4(v[2], v[3], v[4], E), cmn, mx);
+        // point copies: sorted positions 0, 1 -> left child, 2..4 -> right
+        float* rp = a.pts + (size_t)m * 4 * D;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            float pt[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                float x = best[d];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (perm[j] == q) x = p[q][d];
+                pt[d] = x;
+            }
+            float* dst = j < 2 ? lp + j * D : rp + (j - 2) * D;
+#pragma unroll
+            for (int d = 0; d < D; ++d) dst[d] = pt[d];
+        }
         a.err[i] = -1.0f;
         a.err[m] = -1.0f;
         a.seed[m] = 0u;
@@ -583,6 +614,24 @@ __global__ __launch_bounds__(256) void k_place(DevGrid a) {
 template <int D>
 __global__ __launch_bounds__(256) void k_place_b(const DevGrid* grids) {
     place_body<D>(grids[blockIdx.y], blockIdx.x, gridDim.x);
+}
+
+// Per-leaf point copies from the sample coordinates (after a state upload).
+template <int D>
+__global__ __launch_bounds__(256) void k_build_pts(DevGrid a, int L) {
+    constexpr int Q = leaf_quads(D);
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < L; i += gridDim.x * 256) {
+        const int4 id = load_slots(a, i, Q);
+        const int ids[4] = {id.x, id.y, id.z, id.w};
+        float* lp = a.pts + (size_t)i * 4 * D;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (ids[q] >= 0 && ids[q] < a.cap_s) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) lp[q * D + d] = a.coords[(size_t)ids[q] * D + d];
+            }
+        }
+    }
 }
 
 // --------------------------------------------------------------- fills --
@@ -669,6 +718,13 @@ hipError_t launch_adaptive(Grid& g, hipStream_t st) {
     if (e == hipSuccess) e = launch_scan(g, st);
     if (e == hipSuccess) e = launch_place(g, st);
     return e;
+}
+
+hipError_t launch_build_pts(Grid& g, int n_leaves, hipStream_t st) {
+    if (n_leaves <= 0) return hipSuccess;
+    const int blocks = grid_for(n_leaves, 4096);
+    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_build_pts<DD>, dim3(blocks), dim3(256), 0, st, g.dev, n_leaves));
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_i32(int* p, int v, long long n, hipStream_t st) {

@@ -43,7 +43,7 @@ hipError_t dalloc(T** p, size_t n) {
 
 void free_grid(Grid& g) {
     mdas::DevGrid& d = g.dev;
-    void* ptrs[] = {d.coords,   d.values, d.leaf,     d.err,  d.seed,       d.flags,
+    void* ptrs[] = {d.coords,   d.values, d.leaf,     d.err,  d.seed,       d.flags, d.pts,
                     d.split_list, d.part, d.part_off, d.epart, d.rrec,      d.cnt,
                     d.pstats,   g.keys,   g.keys_alt, g.tile_start, g.tile_end, g.rs_hist,
                     g.d_scalar, g.learn.feat, g.learn.perr, g.learn.rows};
@@ -163,6 +163,7 @@ int mdas_create(const mdas_config* cfg, mdas_handle* out) {
     if (e == hipSuccess) e = dalloc(&d.coords, cs * D);
     if (e == hipSuccess) e = dalloc(&d.values, cs * 3);
     if (e == hipSuccess) e = dalloc(&d.leaf, cl * mdas::leaf_quads(D));
+    if (e == hipSuccess) e = dalloc(&d.pts, cl * 4 * D);
     if (e == hipSuccess) e = dalloc(&d.err, cl);
     if (e == hipSuccess) e = dalloc(&d.seed, cl);
     if (e == hipSuccess) e = dalloc(&d.flags, cl);
@@ -451,6 +452,10 @@ int mdas_upload_state(mdas_handle h, int n, int l, const float* coords, const fl
         HIP_TRY(hipMemcpy(g.dev.leaf, rec.data(), sizeof(float) * rec.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(g.dev.flags, f.data(), f.size(), hipMemcpyHostToDevice));
     }
+    if (idx || coords) {
+        HIP_TRY(mdas::launch_build_pts(g, l, 0));
+        HIP_TRY(hipDeviceSynchronize());
+    }
     mdas::DevCounters c{};
     c.n_samples = n;
     c.n_leaves = l;
@@ -713,14 +718,15 @@ int mdas_get_pass_stats(mdas_handle h, int* out, int max_rows) {
 // Algorithmic bytes of each kernel over the last run + reconstruction: the
 // bytes the algorithm must move with this layout, each counted once: FP32
 // coordinates (4*D per sample) and values (12), leaf records of LB = 48 or 64
-// bytes (slots + box, read or written whole), 4-byte errors and LCG states,
-// 1-byte flags, 8-byte keys, 48-byte reconstruction records, float4 pixels.
+// bytes (slots + box, read or written whole), the per-leaf copies of the
+// leaf's points (4*D per point), 4-byte errors and LCG states, 1-byte flags,
+// 8-byte keys, 48-byte reconstruction records, float4 pixels.
 static double kernel_bytes(const Grid& g, const std::string& k) {
     const double D = g.D;
     const double LB = 16.0 * mdas::leaf_quads(g.D);
     const double L0 = (double)(1LL << (g.b * g.D + 2 * g.e));
     double b = 0.0;
-    if (k == "k_initial") return L0 * (16.0 * D + LB + 4.0 + 4.0 + 1.0);
+    if (k == "k_initial") return L0 * (2.0 * 16.0 * D + LB + 4.0 + 4.0 + 1.0);  // + point copies
     if (k == "k_eval") {
         double n = (double)g.host.n_samples;
         return n * (4.0 * D + 12.0);
@@ -735,9 +741,10 @@ static double kernel_bytes(const Grid& g, const std::string& k) {
         else if (k == "k_place")
             b += 1.0 * p.L +                                              // flags
                  (double)p.sel * (4.0 + LB + 4.0 * D + 4.0 + 1.0) +       // state, record, new sample
-                 4.0 * D * p.selcnt +                                     // the leaf's samples
-                 (double)(p.sel - p.split) * 16.0 +                       // slots of an insert
-                 (double)p.split * (2.0 * LB + 8.0 + 4.0 + 2.0 + 4.0);    // two records, errors, ...
+                 4.0 * D * p.selcnt +                                     // the leaf's points
+                 (double)(p.sel - p.split) * (16.0 + 4.0 * D) +           // slots + point of an insert
+                 (double)p.split * (2.0 * LB + 20.0 * D + 8.0 + 4.0 + 2.0 + 4.0);  // two records
+                                                                          // and their points, errors
     }
     const double L = (double)g.host.n_leaves, N = (double)g.host.n_samples;
     const double K = (double)g.last_n_keys;

@@ -58,6 +58,7 @@ struct DevGrid {
     float* coords;         // [cap_s * D]  AoS (the integrand contract)
     float* values;         // [cap_s * 3]  float3 AoS
     float4* leaf;          // [cap_l * leaf_quads(D)]  slots (~D = empty) + box
+    float* pts;            // [cap_l * 4 * D]  copy of each leaf's sample points by slot
     float* err;            // [cap_l]      < 0 = stale
     unsigned* seed;        // [cap_l]      LCG state, 0 = unset
     unsigned char* flags;  // [cap_l]      bits 0-2 count, 3 selected, 4 split
@@ -179,6 +180,7 @@ hipError_t launch_place(Grid& g, hipStream_t st);
 hipError_t launch_pass_begin(Grid& g, hipStream_t st);
 hipError_t launch_set_limit(Grid& g, int limit, hipStream_t st);
 hipError_t launch_arm(Grid& g, hipStream_t st);
+hipError_t launch_build_pts(Grid& g, int n_leaves, hipStream_t st);
 hipError_t launch_fill_i32(int* p, int v, long long n, hipStream_t st);
 hipError_t launch_fill_f32(float* p, float v, long long n, hipStream_t st);
 

@@ -1,0 +1,139 @@
+"""Learned proposal, host side: collect training rows from plain-MDAS runs
+over seeded scenes, train the network on the GPU, and compare learned and
+plain sampling by image error against an FP64 high-spp QMC ground truth.
+
+Every step runs on the device through libmdas (collection mode of mdas_run,
+mdas_mlp_train, scoring mode of mdas_run, mdas_render_qmc); numpy only
+computes the image metrics.
+"""
+import math
+import time
+
+import numpy as np
+
+from . import _lib, hip
+from .kdgrid import KDGrid, Mlp, Scene
+from .metrics import mse, rel_mse
+
+SCENE_MBDOF = _lib.SCENE_SPHERES_MBDOF
+
+
+def extra_bits(W, H, dim, b, spp0=4):
+    """extraImgBits e giving about spp0 initial samples per pixel:
+    4 * 2^(b*dim) * 4^e ~= spp0 * W * H."""
+    return max(0, int(round(math.log(spp0 * W * H / (4.0 * 2 ** (b * dim)), 4))))
+
+
+def make_grid(W, H, spp, dim, b=1, e=None, alpha=0.25, cand=4):
+    e = extra_bits(W, H, dim, b) if e is None else e
+    target = int(spp * W * H)
+    cap = int(target * 1.5) + 4 * (1 << (b * dim + 2 * e)) + 1024
+    return KDGrid(dim, cap, cand, b, e, alpha, float(W), float(H)), target
+
+
+def collect(seeds, W, H, spp, dim=5, kind=SCENE_MBDOF, b=1, cap_rows=1 << 24, log=print):
+    """Rows (device X [n,16], Y [n]) from plain-MDAS runs of the seeded scenes."""
+    X = hip.DeviceBuffer(cap_rows * 64)
+    Y = hip.DeviceBuffer(cap_rows * 4)
+    n = 0
+    for s in seeds:
+        if n >= cap_rows:
+            break
+        g, target = make_grid(W, H, spp, dim, b)
+        sc = Scene(kind, int(s), dim, float(W), float(H))
+        g.set_learned(_lib.LEARN_COLLECT, None, X.ptr + n * 64, Y.ptr + n * 4, cap_rows - n)
+        g.Run(sc, target)
+        n += min(g.learned_rows(), cap_rows - n)
+        g.close()
+        sc.close()
+    log(f"[learned] collected {n} rows from {len(seeds)} scenes ({W}x{H}, {spp} spp, D={dim})")
+    return X, Y, n
+
+
+def train(X, Y, n, epochs=4, batch=65536, lr=1e-3, seed=0, log=print):
+    m = Mlp(seed)
+    m.fit_norm(X, n)
+    t0 = time.perf_counter()
+    losses = m.train(X, Y, n, epochs, batch, lr, seed)
+    log(f"[learned] trained {epochs} epochs x {math.ceil(n / batch)} steps in "
+        f"{time.perf_counter() - t0:.2f} s; epoch losses {np.round(losses, 4).tolist()}")
+    return m, losses
+
+
+def heldout_loss(m, X, Y, n):
+    """(network MSE, constant-predictor MSE) on device rows."""
+    Xh = X.download(np.float32, n * 16).reshape(n, 16)
+    Yh = Y.download(np.float32, n)
+    y = m.forward(Xh)
+    return float(np.mean((y - Yh) ** 2)), float(np.var(Yh))
+
+
+def render(scene, W, H, spp, dim, mlp=None, b=1):
+    """MDAS image (H, W, 4), samples, sampling ms (device) for one scene."""
+    g, target = make_grid(W, H, spp, dim, b)
+    if mlp is not None:
+        g.set_learned(_lib.LEARN_SCORE, mlp)
+    g.set_timing(True)
+    g.Run(scene, target)
+    img = g.integrate_image(W, H)
+    pt = g.phase_times()
+    ms = pt["initial_ms"] + pt["errors_ms"] + pt["adaptive_ms"] + pt["recon_ms"]
+    n = g.GetNumberOfSamples()
+    g.close()
+    return img, n, ms, pt["integrand_ms"]
+
+
+def ground_truth(scene, W, H, spp=1024):
+    return scene.render_qmc(W, H, spp, fp64=True)
+
+
+def compare(mlp, seeds, W, H, spps, dim=5, kind=SCENE_MBDOF, gt_spp=1024, log=print):
+    """Per seed and spp: image MSE / relMSE of plain and learned sampling
+    against the FP64 ground truth, and their sample counts and times."""
+    rows = []
+    for s in seeds:
+        sc = Scene(kind, int(s), dim, float(W), float(H))
+        gt = ground_truth(sc, W, H, gt_spp)
+        for spp in spps:
+            r = dict(seed=int(s), spp=spp)
+            for name, net in (("plain", None), ("learned", mlp)):
+                img, n, ms, ims = render(sc, W, H, spp, dim, net)
+                r[name] = dict(mse=mse(img, gt), rel_mse=rel_mse(img, gt), samples=n, ms=ms,
+                               integrand_ms=ims)
+            log(f"[learned] seed {s} spp {spp}: plain mse {r['plain']['mse']:.4e} "
+                f"learned {r['learned']['mse']:.4e}")
+            rows.append(r)
+        sc.close()
+    return rows
+
+
+def samples_to_equal(rows, ref_spp, key="mse"):
+    """Samples the learned method needs to reach the plain method's mean error
+    at ref_spp, relative to plain's samples there (log-log interpolation of
+    the mean error over seeds); None if the learned curve never gets there."""
+    spps = sorted({r["spp"] for r in rows})
+
+    def curve(name):
+        e = [np.mean([r[name][key] for r in rows if r["spp"] == s]) for s in spps]
+        n = [np.mean([r[name]["samples"] for r in rows if r["spp"] == s]) for s in spps]
+        t = [np.mean([r[name]["ms"] + r[name]["integrand_ms"] for r in rows if r["spp"] == s])
+             for s in spps]
+        return np.array(n), np.array(e), np.array(t)
+
+    n_p, e_p, t_p = curve("plain")
+    n_l, e_l, t_l = curve("learned")
+    k = spps.index(ref_spp)
+    target = e_p[k]
+    le, ln = np.log(e_l), np.log(n_l)
+    for j in range(len(spps) - 1):
+        a, b = le[j], le[j + 1]
+        if (a - np.log(target)) * (b - np.log(target)) <= 0 and a != b:
+            w = (np.log(target) - a) / (b - a)
+            n_eq = math.exp(ln[j] + w * (ln[j + 1] - ln[j]))
+            t_eq = math.exp(np.log(t_l[j]) + w * (np.log(t_l[j + 1]) - np.log(t_l[j])))
+            return dict(sample_ratio=n_eq / n_p[k], time_ratio=t_eq / t_p[k],
+                        plain_err=float(target), plain_samples=float(n_p[k]))
+    if le[0] <= np.log(target):
+        return dict(sample_ratio=float(n_l[0] / n_p[k]), time_ratio=float(t_l[0] / t_p[k]),
+                    plain_err=float(target), plain_samples=float(n_p[k]), bound="upper")
+    return None
