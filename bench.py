@@ -20,6 +20,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -156,6 +158,57 @@ def run_batch(steps, log=print):
     return res
 
 
+LEARNED = dict(train_seeds=200, train_W=160, train_H=90, spp=32, epochs=4, batch=65536, lr=1e-3,
+               val_seeds=(1000, 1001, 1002, 1003), val_W=240, val_H=135,
+               kappas=(1.0, 0.5, 0.25, 0.125, 0.0625),
+               eval_seeds=(2000, 2001, 2002, 2003), eval_W=480, eval_H=270,
+               eval_spp=(8, 12, 16, 24, 32, 64), gt_spp=4096)
+
+
+def learned_comparison(log=print):
+    """Learned proposal vs plain MDAS on the 5D moving-sphere scene: rows
+    from plain runs of 200 seeded scenes, GPU training, the score exponent
+    chosen on 4 validation scenes, then image MSE against an FP64 4096-spp
+    QMC ground truth on 4 held-out scenes; samples and time the learned form
+    needs to reach plain's error at 32 spp, and the same for plain sampling
+    with the learned form's effective exponent (the flattening alone)."""
+    from pmdas_amd import learned as L
+    c = LEARNED
+    t0 = time.perf_counter()
+    X, Y, n = L.collect(range(c["train_seeds"]), c["train_W"], c["train_H"], c["spp"],
+                        cap_rows=1 << 25, log=log)
+    Xh, Yh, nh = L.collect(range(1000, 1010), c["train_W"], c["train_H"], c["spp"],
+                           cap_rows=1 << 22, log=log)
+    m, losses = L.train(X, Y, n, c["epochs"], c["batch"], c["lr"], seed=0, log=log)
+    net_loss, const_loss = L.heldout_loss(m, Xh, Yh, nh)
+    kappa, ktable = L.tune_kappa(m, c["val_seeds"], c["val_W"], c["val_H"], 32, c["kappas"],
+                                 gt_spp=c["gt_spp"], log=log)
+    rows = L.compare(m, c["eval_seeds"], c["eval_W"], c["eval_H"], c["eval_spp"],
+                     gt_spp=c["gt_spp"], kappa=kappa, log=log)
+    arms = ("plain", "learned", "plain_same_exponent")
+    eq = {arm: {k: L.samples_to_equal(rows, 32, k, arm) for k in ("mse", "rel_mse")}
+          for arm in arms[1:]}
+    per_spp = {}
+    for spp in c["eval_spp"]:
+        sel = [r for r in rows if r["spp"] == spp]
+        per_spp[str(spp)] = {k: {"mse": float(np.mean([r[k]["mse"] for r in sel])),
+                                 "rel_mse": float(np.mean([r[k]["rel_mse"] for r in sel])),
+                                 "samples": float(np.mean([r[k]["samples"] for r in sel])),
+                                 "ms": float(np.mean([r[k]["ms"] + r[k]["integrand_ms"]
+                                                      for r in sel]))}
+                             for k in arms}
+    for b in (X, Y, Xh, Yh):
+        b.free()
+    m.close()
+    return {"train_rows": n, "epoch_loss": [round(float(v), 5) for v in losses],
+            "heldout_loss": round(net_loss, 5), "heldout_const_loss": round(const_loss, 5),
+            "kappa": kappa, "validation_mse_by_kappa": {str(k): v for k, v in ktable.items()},
+            "eval": f"{len(c['eval_seeds'])} held-out seeds, {c['eval_W']}x{c['eval_H']} 5D, "
+                    f"FP64 QMC {c['gt_spp']} spp ground truth",
+            "per_spp": per_spp, "equal_error_at_32spp": eq,
+            "seconds": round(time.perf_counter() - t0, 1)}
+
+
 def roofline(res, traffic_file):
     """Dominant MDAS kernel (integrand excluded): algorithmic bytes / its time."""
     ks = {k: v for k, v in res["kernels"].items() if k != "k_eval"}
@@ -229,6 +282,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--cpu-shrink", type=int, default=0,
                     help="run the CPU baseline on a 2^-k per-axis version of the workload")
     a = ap.parse_args()
@@ -287,6 +341,8 @@ def main():
                              "samples": r2["samples"], "passes": r2["passes"]}
     if not a.no_batch:
         line["batch"] = run_batch(max(1, min(2, a.steps)), log)
+    if not a.no_learned:
+        line["learned"] = learned_comparison(log)
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_shrink, log)
     else:

@@ -241,6 +241,10 @@ typedef struct mdas_mlp* mdas_mlp_handle;
 #define MDAS_FEATURES 16
 #define MDAS_LEARN_COLLECT 1 /* write (parent features, label) rows of every split */
 #define MDAS_LEARN_SCORE 2   /* select by the network's score */
+/* Second label form: log2 of the squared image change a split makes,
+ * sum over children c of (m_c - m_p)^2 v_c^2 / a_c (m mean luminance, v box
+ * volume, a xy footprint area); scored as 2^clamp(prediction, -120, 120). */
+#define MDAS_LEARN_IMAGE_LABEL 4
 
 /* Seeded default initialisation: U(-1/sqrt(fan_in), 1/sqrt(fan_in)). */
 int mdas_mlp_create(uint32_t seed, mdas_mlp_handle* out);
@@ -271,6 +275,10 @@ int mdas_mlp_train(mdas_mlp_handle m, const float* rows, const float* targets, l
  * Resets the row counter. */
 int mdas_set_learned(mdas_handle h, int mode, mdas_mlp_handle m, float* rows, float* labels,
                      long long cap_rows);
+/* Score exponent kappa in (0, 16] (default 1, reset by mdas_set_learned):
+ * the network's score s becomes s^kappa, i.e. selection tests
+ * pow(s / max s, kappa * alpha) >= u -- the learned form's own flattening. */
+int mdas_set_learned_exponent(mdas_handle h, float kappa);
 /* Rows offered since mdas_set_learned (those beyond cap_rows were dropped). */
 int mdas_get_learned_rows(mdas_handle h, long long* offered);
 /* Features and plain errors of leaves [0, n) after a learned-mode run. */

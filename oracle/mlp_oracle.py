@@ -161,6 +161,18 @@ def split_label(ep, el, er):
     return float(np.log2(np.float64(r)))
 
 
+def image_label(mp, mc, vc, ac):
+    """log2(sum_c (m_c - m_p)^2 v_c^2 / a_c + 1e-30) in the kernel's float32
+    operation order (log2 in float64)."""
+    f32 = np.float32
+    s = f32(0.0)
+    for h in range(2):
+        d = f32(f32(mc[h]) - f32(mp))
+        w = f32(f32(f32(vc[h]) * f32(vc[h])) / f32(ac[h]))
+        s = f32(s + f32(f32(d * d) * w))
+    return float(np.log2(np.float64(f32(s + f32(1e-30)))))
+
+
 class CollectTracker:
     """Replays the collection mode on the grid oracle.  After each
     compute_errors call after_errors(state, stale leaves): it computes the
@@ -168,8 +180,9 @@ class CollectTracker:
     the previous pass, in split order.  After each adaptive pass call
     after_adaptive(state before, state after) to record that pass's splits."""
 
-    def __init__(self, D, sX, sY):
+    def __init__(self, D, sX, sY, label="reduction"):
         self.D, self.sX, self.sY = D, sX, sY
+        self.label = label
         self.feat = None
         self.depth = None
         self.perr = None
@@ -212,7 +225,21 @@ class CollectTracker:
             for j, i in enumerate(self.parents):
                 m = self.L_old + j
                 rows_X.append(self.feat[i].copy())
-                rows_Y.append(split_label(self.perr[i], st["err"][i], st["err"][m]))
+                if self.label == "reduction":
+                    rows_Y.append(split_label(self.perr[i], st["err"][i], st["err"][m]))
+                else:
+                    mc, vc, ac = [], [], []
+                    for c in (i, m):
+                        vals, n, mn, mx = self._leaf(st, c)
+                        fc = leaf_features(vals, n, mn, mx, st["err"][c], 0, self.D, self.sX,
+                                           self.sY)
+                        mc.append(np.float32(fc[7]))
+                        vol = np.float32(1.0)
+                        for d in range(self.D):
+                            vol = np.float32(vol * np.float32(np.float32(mx[d]) - np.float32(mn[d])))
+                        vc.append(vol)
+                        ac.append(np.float32(np.float32(mx[0] - mn[0]) * np.float32(mx[1] - mn[1])))
+                    rows_Y.append(image_label(np.float32(self.feat[i][7]), mc, vc, ac))
             feat = np.zeros((L, 16))
             feat[:len(self.feat)] = self.feat
             perr = np.zeros(L, np.float32)

@@ -11,7 +11,11 @@
 // measured by the plain error estimate of the two children one pass later.
 // Score used for selection: e * 2^clamp(y), y the network's prediction of
 // log2 r; the plain error e is the fallback when the network is absent or its
-// output is not finite.
+// output is not finite.  Second label form: log2 of the squared image change
+// the split makes (image_label below), scored as 2^clamp(y).  A score
+// exponent kappa (default 1) replaces the score by score^kappa, so the
+// selection test pow(score / max, alpha) uses the effective exponent
+// kappa * alpha.
 #include "mdas_leaf.hpp"
 #include "mlp_device.hpp"
 
@@ -71,6 +75,23 @@ __device__ __forceinline__ float split_label(float ep, float el, float er) {
     return __log2f(r);
 }
 
+// Image-change label: the reconstruction spreads a leaf's mean times its
+// volume over its xy footprint, so refining parent p into children c moves
+// the image by (m_c - m_p) v_c / a_c over each child's footprint a_c; the
+// label is log2 of the squared change integrated over the footprints,
+// sum_c (m_c - m_p)^2 v_c^2 / a_c (m = mean luminance, v = volume).
+__device__ __forceinline__ float image_label(float mp, const float* mc, const float* vc,
+                                             const float* ac) {
+    float s = 0.0f;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const float d = mc[h] - mp;
+        const float w = vc[h] * vc[h] / ac[h];
+        s += d * d * w;
+    }
+    return __log2f(s + 1.0e-30f);
+}
+
 // ---------------------------------------------------- fused error pass --
 // Same leaves as the plain stale-error pass (every leaf on the first pass,
 // afterwards the two children of each split, handled here as one item so the
@@ -111,7 +132,7 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
             depth = pf[11] + 1.0f;
             pe = l.perr[c[0]];
         }
-        float e[2] = {0.0f, 0.0f};
+        float e[2] = {0.0f, 0.0f}, vol[2] = {1.0f, 1.0f}, axy[2] = {1.0f, 1.0f};
         float f[2][kFeat];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -120,6 +141,8 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
                 float mn[D], mx[D];
                 e[h] = leaf_error_s<D>(a, c[h], ls, mn, mx);
                 leaf_features<D>(ls, mn, mx, e[h], depth, a.sX, a.sY, f[h]);
+                vol[h] = ls.vol;
+                axy[h] = (mx[0] - mn[0]) * (mx[1] - mn[1]);
                 n_stale += 1;
                 n_vals += ls.n;
             } else {
@@ -132,7 +155,12 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
             if (row < l.cap_rows) {
 #pragma unroll
                 for (int k = 0; k < kFeat; ++k) l.X[row * kFeat + k] = pf[k];
-                l.Y[row] = split_label(pe, e[0], e[1]);
+                if (l.label_mode == 0) {
+                    l.Y[row] = split_label(pe, e[0], e[1]);
+                } else {
+                    const float mc[2] = {f[0][7], f[1][7]};
+                    l.Y[row] = image_label(pf[7], mc, vol, axy);
+                }
             }
         }
 #pragma unroll
@@ -163,7 +191,12 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
                     if ((lane >> 4) == tile && c[h] >= 0) {
                         const float yv = ys[w][lane & 15];
                         if (fabsf(yv) < 3.0e38f) {  // false for NaN and +-inf
-                            const float s = e[h] * exp2f(fminf(fmaxf(yv, l.ylo), l.yhi));
+                            const float yc = fminf(fmaxf(yv, l.ylo), l.yhi);
+                            float s;
+                            if (l.kappa == 1.0f)
+                                s = l.label_mode == 0 ? e[h] * exp2f(yc) : exp2f(yc);
+                            else  // (score)^kappa
+                                s = exp2f(l.kappa * (l.label_mode == 0 ? log2_floor(e[h]) + yc : yc));
                             if (s < 3.0e38f) q[h] = s;
                         }
                     }

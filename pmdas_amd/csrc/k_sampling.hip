@@ -283,7 +283,7 @@ __device__ __forceinline__ void select_body(const DevGrid& a, int blk, int nblk)
         unsigned s = a.seed[i];
         if (s == 0u) s = tea4((unsigned)i, 0u);
         const float u = rnd(s);
-        const bool sel = powf_mdas(e / gmax, alpha) >= u;
+        const bool sel = select_test(e / gmax, alpha, u);
         const unsigned cnt = a.flags[i] & 7u;
         const bool split = sel && cnt == 4u;
         a.flags[i] = (unsigned char)(cnt | (sel ? 8u : 0u) | (split ? 16u : 0u));

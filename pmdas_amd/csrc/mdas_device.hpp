@@ -92,6 +92,23 @@ __device__ __forceinline__ float powf_mdas(float x, float a) {
     return (float)exp_dbl(y);
 }
 
+// The selection test powf_mdas(x, a) >= u (kdgrid.cu:326) with a fast path:
+// p = exp2(a * log2 x) from the hardware log / exp (relative error below
+// ~1e-5 for |a log2 x| < 64) decides whenever it is more than 2^-10 away
+// from u relative -- then the exactly rounded pow lies on the same side of u
+// -- and only the rest evaluates powf_mdas.  Same result bit for bit.
+__device__ __forceinline__ bool select_test(float x, float a, float u) {
+    if (x >= 1.1754944e-38f && x <= 1.0f && a > 0.0f && u > 0.0f) {
+        const float t = a * __builtin_amdgcn_logf(x);
+        if (t > -64.0f) {
+            const float p = __builtin_amdgcn_exp2f(t);
+            if (p >= u * 1.0009765625f) return true;
+            if (p <= u * 0.9990234375f) return false;
+        }
+    }
+    return powf_mdas(x, a) >= u;
+}
+
 // kdgrid.cu:141-164 -- leaf index -> cell offset / extent of the initial grid.
 // Bit k < b of coordinate d is index bit D*k + (D-1-d); for the image
 // dimensions (d < 2) bit k in [b, b+e) is index bit D*b + 2(k-b) + 1 - d.

@@ -100,7 +100,9 @@ struct LearnDev {
     long long* rows;     // rows offered so far (index >= cap_rows: dropped)
     long long cap_rows;
     MlpDev mlp;          // mlp.p == null: the score is the plain error
-    float ylo, yhi;      // clamp of the predicted log2 reduction ratio
+    float ylo, yhi;      // clamp of the network output
+    int label_mode;      // 0: relative error reduction, score e 2^y; 1: image change, score 2^y
+    float kappa;         // score exponent (1: the score itself)
 };
 
 struct Grid {

@@ -256,7 +256,7 @@ int mdas_mlp_train(mdas_mlp_handle m, const float* rows, const float* targets, l
 int mdas_set_learned(mdas_handle h, int mode, mdas_mlp_handle m, float* rows, float* labels,
                      long long cap_rows) {
     if (!h) return set_error(MDAS_ERR_INVALID, "null handle");
-    if (mode & ~(MDAS_LEARN_COLLECT | MDAS_LEARN_SCORE))
+    if (mode & ~(MDAS_LEARN_COLLECT | MDAS_LEARN_SCORE | MDAS_LEARN_IMAGE_LABEL))
         return set_error(MDAS_ERR_INVALID, "unknown learned mode bits");
     if ((mode & MDAS_LEARN_SCORE) && !m)
         return set_error(MDAS_ERR_INVALID, "scoring needs a network");
@@ -274,9 +274,18 @@ int mdas_set_learned(mdas_handle h, int mode, mdas_mlp_handle m, float* rows, fl
     g.learn.Y = (mode & MDAS_LEARN_COLLECT) ? labels : nullptr;
     g.learn.cap_rows = (mode & MDAS_LEARN_COLLECT) ? cap_rows : 0;
     g.learn.mlp = (mode & MDAS_LEARN_SCORE) ? m->dev() : mdas::MlpDev{};
-    g.learn.ylo = -10.0f;
-    g.learn.yhi = 0.0f;
+    g.learn.label_mode = (mode & MDAS_LEARN_IMAGE_LABEL) ? 1 : 0;
+    g.learn.kappa = 1.0f;
+    g.learn.ylo = g.learn.label_mode ? -120.0f : -10.0f;
+    g.learn.yhi = g.learn.label_mode ? 120.0f : 0.0f;
     if (g.learn.rows) HIP_OK(hipMemset(g.learn.rows, 0, sizeof(long long)));
+    return MDAS_OK;
+}
+
+int mdas_set_learned_exponent(mdas_handle h, float kappa) {
+    if (!h) return set_error(MDAS_ERR_INVALID, "null handle");
+    if (!(kappa > 0.0f) || kappa > 16.0f) return set_error(MDAS_ERR_INVALID, "kappa must be in (0, 16]");
+    h->g.learn.kappa = kappa;
     return MDAS_OK;
 }
 

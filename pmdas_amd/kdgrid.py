@@ -308,6 +308,9 @@ class KDGrid:
         check(_lib.load().mdas_set_learned(self._h, mode, None if mlp is None else mlp.handle,
                                            ptr(rows), ptr(labels), cap_rows))
 
+    def set_learned_exponent(self, kappa):
+        check(_lib.load().mdas_set_learned_exponent(self._h, kappa))
+
     def learned_rows(self):
         n = ctypes.c_longlong(0)
         check(_lib.load().mdas_get_learned_rows(self._h, ctypes.byref(n)))

@@ -31,8 +31,10 @@ def make_grid(W, H, spp, dim, b=1, e=None, alpha=0.25, cand=4):
     return KDGrid(dim, cap, cand, b, e, alpha, float(W), float(H)), target
 
 
-def collect(seeds, W, H, spp, dim=5, kind=SCENE_MBDOF, b=1, cap_rows=1 << 24, log=print):
-    """Rows (device X [n,16], Y [n]) from plain-MDAS runs of the seeded scenes."""
+def collect(seeds, W, H, spp, dim=5, kind=SCENE_MBDOF, b=1, cap_rows=1 << 24, image_label=False,
+            log=print):
+    """Rows (device X [n,16], Y [n]) from plain-MDAS runs of the seeded scenes
+    (labels: relative error reduction, or the image change if image_label)."""
     X = hip.DeviceBuffer(cap_rows * 64)
     Y = hip.DeviceBuffer(cap_rows * 4)
     n = 0
@@ -41,7 +43,8 @@ def collect(seeds, W, H, spp, dim=5, kind=SCENE_MBDOF, b=1, cap_rows=1 << 24, lo
             break
         g, target = make_grid(W, H, spp, dim, b)
         sc = Scene(kind, int(s), dim, float(W), float(H))
-        g.set_learned(_lib.LEARN_COLLECT, None, X.ptr + n * 64, Y.ptr + n * 4, cap_rows - n)
+        mode = _lib.LEARN_COLLECT | (_lib.LEARN_IMAGE_LABEL if image_label else 0)
+        g.set_learned(mode, None, X.ptr + n * 64, Y.ptr + n * 4, cap_rows - n)
         g.Run(sc, target)
         n += min(g.learned_rows(), cap_rows - n)
         g.close()
@@ -68,11 +71,13 @@ def heldout_loss(m, X, Y, n):
     return float(np.mean((y - Yh) ** 2)), float(np.var(Yh))
 
 
-def render(scene, W, H, spp, dim, mlp=None, b=1):
+def render(scene, W, H, spp, dim, mlp=None, b=1, image_label=False, alpha=0.25, kappa=1.0):
     """MDAS image (H, W, 4), samples, sampling ms (device) for one scene."""
-    g, target = make_grid(W, H, spp, dim, b)
+    g, target = make_grid(W, H, spp, dim, b, alpha=alpha)
     if mlp is not None:
-        g.set_learned(_lib.LEARN_SCORE, mlp)
+        g.set_learned(_lib.LEARN_SCORE | (_lib.LEARN_IMAGE_LABEL if image_label else 0), mlp)
+        if kappa != 1.0:
+            g.set_learned_exponent(kappa)
     g.set_timing(True)
     g.Run(scene, target)
     img = g.integrate_image(W, H)
@@ -83,34 +88,60 @@ def render(scene, W, H, spp, dim, mlp=None, b=1):
     return img, n, ms, pt["integrand_ms"]
 
 
-def ground_truth(scene, W, H, spp=1024):
+def ground_truth(scene, W, H, spp=4096):
     return scene.render_qmc(W, H, spp, fp64=True)
 
 
-def compare(mlp, seeds, W, H, spps, dim=5, kind=SCENE_MBDOF, gt_spp=1024, log=print):
-    """Per seed and spp: image MSE / relMSE of plain and learned sampling
-    against the FP64 ground truth, and their sample counts and times."""
+def tune_kappa(mlp, seeds, W, H, spp, kappas, dim=5, kind=SCENE_MBDOF, gt_spp=4096,
+               image_label=False, alpha=0.25, log=print):
+    """Score exponent with the lowest mean image MSE on validation scenes
+    (seeds disjoint from the evaluation)."""
+    scenes = [Scene(kind, int(s), dim, float(W), float(H)) for s in seeds]
+    gts = [ground_truth(sc, W, H, gt_spp) for sc in scenes]
+    best, table = None, {}
+    for k in kappas:
+        e = float(np.mean([mse(render(sc, W, H, spp, dim, mlp, image_label=image_label,
+                                      alpha=alpha, kappa=k)[0], gt)
+                           for sc, gt in zip(scenes, gts)]))
+        table[k] = e
+        if best is None or e < table[best]:
+            best = k
+    for sc in scenes:
+        sc.close()
+    log(f"[learned] validation mse by kappa {table}; chose {best}")
+    return best, table
+
+
+def compare(mlp, seeds, W, H, spps, dim=5, kind=SCENE_MBDOF, gt_spp=4096, image_label=False,
+            kappa=1.0, alpha=0.25, log=print):
+    """Per seed and spp: image MSE / relMSE against the FP64 ground truth,
+    sample counts and times of plain sampling (alpha), learned sampling
+    (score exponent kappa) and plain sampling with the learned form's
+    effective exponent kappa * alpha (the same flattening without the
+    network)."""
     rows = []
+    arms = (("plain", None, alpha), ("learned", mlp, alpha), ("plain_same_exponent", None, kappa * alpha))
     for s in seeds:
         sc = Scene(kind, int(s), dim, float(W), float(H))
         gt = ground_truth(sc, W, H, gt_spp)
         for spp in spps:
             r = dict(seed=int(s), spp=spp)
-            for name, net in (("plain", None), ("learned", mlp)):
-                img, n, ms, ims = render(sc, W, H, spp, dim, net)
+            for name, net, a in arms:
+                img, n, ms, ims = render(sc, W, H, spp, dim, net, image_label=image_label,
+                                         alpha=a, kappa=kappa)
                 r[name] = dict(mse=mse(img, gt), rel_mse=rel_mse(img, gt), samples=n, ms=ms,
                                integrand_ms=ims)
-            log(f"[learned] seed {s} spp {spp}: plain mse {r['plain']['mse']:.4e} "
-                f"learned {r['learned']['mse']:.4e}")
+            log(f"[learned] seed {s} spp {spp}: mse plain {r['plain']['mse']:.4e} learned "
+                f"{r['learned']['mse']:.4e} plain(alpha*kappa) {r['plain_same_exponent']['mse']:.4e}")
             rows.append(r)
         sc.close()
     return rows
 
 
-def samples_to_equal(rows, ref_spp, key="mse"):
-    """Samples the learned method needs to reach the plain method's mean error
-    at ref_spp, relative to plain's samples there (log-log interpolation of
-    the mean error over seeds); None if the learned curve never gets there."""
+def samples_to_equal(rows, ref_spp, key="mse", arm="learned"):
+    """Samples (and time) an arm needs to reach the plain method's mean error
+    at ref_spp, relative to plain's there (log-log interpolation of the mean
+    error over seeds); None if the arm's curve never gets there."""
     spps = sorted({r["spp"] for r in rows})
 
     def curve(name):
@@ -121,7 +152,7 @@ def samples_to_equal(rows, ref_spp, key="mse"):
         return np.array(n), np.array(e), np.array(t)
 
     n_p, e_p, t_p = curve("plain")
-    n_l, e_l, t_l = curve("learned")
+    n_l, e_l, t_l = curve(arm)
     k = spps.index(ref_spp)
     target = e_p[k]
     le, ln = np.log(e_l), np.log(n_l)

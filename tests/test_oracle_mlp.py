@@ -152,3 +152,14 @@ def test_split_label_closed_form():
     assert mo.split_label(1.0, 0.75, 0.5) == -10.0          # no reduction: the floor
     assert mo.split_label(0.5, 0.0, 0.0) == 0.0             # all removed
     assert mo.split_label(0.0, 0.0, 0.0) == -10.0
+
+
+def test_image_label_closed_form():
+    # one child changes mean by 0.5 with volume 2 over area 4: 0.25 * 4 / 4
+    assert mo.image_label(1.0, [1.5, 1.0], [2.0, 3.0], [4.0, 3.0]) == np.log2(np.float32(0.25 + 1e-30))
+    # both children equal the parent: only the 1e-30 floor remains
+    assert abs(mo.image_label(0.3, [0.3, 0.3], [1.0, 1.0], [1.0, 1.0]) - np.log2(1e-30)) < 1e-6
+    # scaling the volumes by k scales the change by k^2
+    a = mo.image_label(0.0, [1.0, 2.0], [1.0, 1.0], [1.0, 1.0])
+    b = mo.image_label(0.0, [1.0, 2.0], [2.0, 2.0], [1.0, 1.0])
+    assert abs((b - a) - 2.0) < 1e-6
