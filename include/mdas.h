@@ -160,6 +160,11 @@ int mdas_kernel_stats(mdas_handle h, int index, char* name, int name_cap, int* l
 /* Tile of the reconstruction gather: tile_w x tile_h pixels, 16 or 64 in
  * all (default 8 x 8).  The result does not depend on it. */
 int mdas_set_recon_tile(mdas_handle h, int tile_w, int tile_h);
+/* Workgroups of the per-pass kernels: leaf chunks of the selection,
+ * compaction and reconstruction kernels (64..32768, default 4096) and of
+ * the error pass (64..8192, default 1024).  Results do not depend on them
+ * (allocation follows leaf order); only the time does. */
+int mdas_set_launch_shape(mdas_handle h, int chunk_blocks, int err_blocks);
 /* Reconstructs the current state into `pixels` with each candidate tile
  * shape, keeps the fastest and returns it. */
 int mdas_autotune_recon(mdas_handle h, float* pixels, int width, int height, void* stream,

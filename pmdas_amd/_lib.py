@@ -133,6 +133,7 @@ SIGNATURES = {
     "mdas_kernel_stats": (_I, [_H, _I, ctypes.c_char_p, _I, ctypes.POINTER(_I),
                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "mdas_set_recon_tile": (_I, [_H, _I, _I]),
+    "mdas_set_launch_shape": (_I, [_H, _I, _I]),
     "mdas_autotune_recon": (_I, [_H, _P, _I, _I, _P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "mdas_download_state": (_I, [_H, _P, _P, _P, _P, _P, _P]),
     "mdas_upload_state": (_I, [_H, _I, _I, _P, _P, _P, _P, _P, _P, _F]),

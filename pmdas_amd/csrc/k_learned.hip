@@ -277,13 +277,13 @@ __global__ __launch_bounds__(1024) void k_err_reduce_learned(DevGrid a, LearnDev
 
 hipError_t launch_errors_learned(Grid& g, hipStream_t st) {
     switch (g.D) {
-    case 3: hipLaunchKernelGGL(k_errors_learned<3>, dim3(kErrBlocks), dim3(256), 0, st, g.dev, g.learn); break;
-    case 4: hipLaunchKernelGGL(k_errors_learned<4>, dim3(kErrBlocks), dim3(256), 0, st, g.dev, g.learn); break;
-    case 5: hipLaunchKernelGGL(k_errors_learned<5>, dim3(kErrBlocks), dim3(256), 0, st, g.dev, g.learn); break;
-    case 6: hipLaunchKernelGGL(k_errors_learned<6>, dim3(kErrBlocks), dim3(256), 0, st, g.dev, g.learn); break;
+    case 3: hipLaunchKernelGGL(k_errors_learned<3>, dim3(g.err_blocks), dim3(256), 0, st, g.dev, g.learn); break;
+    case 4: hipLaunchKernelGGL(k_errors_learned<4>, dim3(g.err_blocks), dim3(256), 0, st, g.dev, g.learn); break;
+    case 5: hipLaunchKernelGGL(k_errors_learned<5>, dim3(g.err_blocks), dim3(256), 0, st, g.dev, g.learn); break;
+    case 6: hipLaunchKernelGGL(k_errors_learned<6>, dim3(g.err_blocks), dim3(256), 0, st, g.dev, g.learn); break;
     default: return hipErrorInvalidValue;
     }
-    hipLaunchKernelGGL(k_err_reduce_learned, dim3(1), dim3(1024), 0, st, g.dev, g.learn, kErrBlocks);
+    hipLaunchKernelGGL(k_err_reduce_learned, dim3(1), dim3(1024), 0, st, g.dev, g.learn, g.err_blocks);
     return hipGetLastError();
 }
 

@@ -551,11 +551,11 @@ hipError_t recon_prepare(Grid& g, int W, int H, hipStream_t st) {
         }
     }
     kbeg(g, st, "k_recon_prep");
-    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_recon_prep<DD>, dim3(kChunkBlocks), dim3(256), 0, st,
+    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_recon_prep<DD>, dim3(g.chunk_blocks), dim3(256), 0, st,
                                             g.dev, G));
     kend(g, st);
     kbeg(g, st, "k_scan_keys");
-    hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, st, g.dev, kChunkBlocks);
+    hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, st, g.dev, g.chunk_blocks);
     kend(g, st);
     int nk = 0;
     hipError_t e = hipMemcpyAsync(&nk, &g.dev.cnt->n_keys, sizeof(int), hipMemcpyDeviceToHost, st);
@@ -596,7 +596,7 @@ hipError_t launch_integrate_gather(Grid& g, float* pixels, int W, int H, hipStre
                                    hipEvent_t ev_sort0, hipEvent_t ev_sort1) {
     ReconGeo G = make_geo(g, W, H);
     kbeg(g, st, "k_recon_emit");
-    hipLaunchKernelGGL(k_recon_emit, dim3(kChunkBlocks), dim3(256), 0, st, g.dev, G, g.keys,
+    hipLaunchKernelGGL(k_recon_emit, dim3(g.chunk_blocks), dim3(256), 0, st, g.dev, G, g.keys,
                        g.keys_cap);
     kend(g, st);
     if (ev_sort0) (void)hipEventRecord(ev_sort0, st);

@@ -287,6 +287,10 @@ __device__ __forceinline__ void select_body(const DevGrid& a, int blk, int nblk)
         const unsigned cnt = a.flags[i] & 7u;
         const bool split = sel && cnt == 4u;
         a.flags[i] = (unsigned char)(cnt | (sel ? 8u : 0u) | (split ? 16u : 0u));
+        // a selected leaf's state after the placement's candidatesNum * D
+        // draws, by the LCG's affine jump-ahead; the placement recovers the
+        // state it starts from with the inverse map
+        if (sel) s = a.jmul * s + a.jadd;
         a.seed[i] = s;
         ns += sel ? 1 : 0;
         nl += split ? 1 : 0;
@@ -413,7 +417,7 @@ template <int D>
 __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m, unsigned cnt,
                                            int L) {
     constexpr int Q = leaf_quads(D);
-    unsigned s = a.seed[i];
+    unsigned s = a.jinv * (a.seed[i] - a.jadd);
     int4 id4;
     float mn[D], mx[D], diag[D];
     load_leaf<D>(a, i, id4, mn, mx);
@@ -481,7 +485,6 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
         store_slots(a, i, Q, n4);
 #pragma unroll
         for (int d = 0; d < D; ++d) lp[cnt * D + d] = best[d];
-        a.flags[i] = (unsigned char)(cnt + 1u);
     } else {
         // kdgrid.cu:399-471: split.  LargestAxis scans from the last axis down
         // with a strict '<' (aabb.h:34-45).
@@ -559,51 +562,56 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
         a.err[i] = -1.0f;
         a.err[m] = -1.0f;
         a.seed[m] = 0u;
-        a.flags[i] = 2;
         a.flags[m] = 3;
         a.split_list[m - L] = i;  // the split leaves in leaf order (right child m)
     }
-    a.seed[i] = s;
+    // the leaf's own state (already jumped) and count were written by the
+    // selection and compaction kernels
 }
 
-// Selected leaves of a workgroup's chunk are ranked in leaf order and queued
-// in LDS; whenever the queue holds more than three waves' worth they are
-// placed with all lanes busy (the selection rate is typically 20-60 %).
-template <int D>
-__device__ __forceinline__ void place_body(const DevGrid& a, int blk, int nblk) {
+// Compaction: the selected leaves of a workgroup's chunk (the selection's
+// partition) are ranked in leaf order by wave ballots; entry r of the
+// placement list (r = the leaf's rank among all selected leaves, so its new
+// sample is N_old + r) gets (leaf, sample, right child or -1, count).  Every
+// leaf's count is rewritten here as it stands after the placement, in full
+// coalesced lines.
+__device__ __forceinline__ void compact_body(const DevGrid& a, int blk, int nblk) {
     __shared__ int lds[8];
-    __shared__ int q_leaf[1024], q_k[1024], q_m[1024];
-    __shared__ unsigned char q_cnt[1024];
     if (a.cnt->done) return;
     const int L = a.cnt->n_leaves_old;
     const int N0 = a.cnt->n_samples_old;
     int beg, end;
     chunk_range(L, nblk, blk, beg, end);
     int2 base = a.part_off[blk];
-    int qn = 0;
     for (int t0 = beg; t0 < end; t0 += 256) {
         const int i = t0 + threadIdx.x;
         const unsigned f = i < end ? a.flags[i] : 0u;
         const bool sel = (f & 8u) != 0u, split = (f & 16u) != 0u;
+        const unsigned cnt = f & 7u;
         int rs, rl, ts, tl;
         block_rank2(sel, split, lds, rs, rl, ts, tl);
-        if (sel) {
-            const int slot = qn + rs;
-            q_leaf[slot] = i;
-            q_k[slot] = N0 + base.x + rs;
-            q_m[slot] = split ? L + base.y + rl : -1;
-            q_cnt[slot] = (unsigned char)(f & 7u);
-        }
+        if (sel)
+            a.qlist[base.x + rs] = make_int4(i, N0 + base.x + rs, split ? L + base.y + rl : -1, (int)cnt);
+        if (i < end) a.flags[i] = (unsigned char)(split ? 2u : (sel ? cnt + 1u : cnt));
         base.x += ts;
         base.y += tl;
-        qn += ts;
-        if (qn > 768 || t0 + 256 >= end) {
-            __syncthreads();
-            for (int j = threadIdx.x; j < qn; j += 256)
-                place_leaf<D>(a, q_leaf[j], q_k[j], q_m[j], q_cnt[j], L);
-            qn = 0;
-            __syncthreads();
-        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_compact(DevGrid a) { compact_body(a, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(256) void k_compact_b(const DevGrid* grids) {
+    compact_body(grids[blockIdx.y], blockIdx.x, gridDim.x);
+}
+
+// Placement: one thread per list entry, every lane busy.
+template <int D>
+__device__ __forceinline__ void place_body(const DevGrid& a, int blk, int nblk) {
+    if (a.cnt->done) return;
+    const int L = a.cnt->n_leaves_old;
+    const int n = a.cnt->new_samples;
+    for (int j = blk * 256 + threadIdx.x; j < n; j += nblk * 256) {
+        const int4 q = a.qlist[j];
+        place_leaf<D>(a, q.x, q.y, q.z, (unsigned)q.w, L);
     }
 }
 
@@ -687,29 +695,30 @@ hipError_t launch_arm(Grid& g, hipStream_t st) {
 }
 
 hipError_t launch_errors(Grid& g, hipStream_t st) {
-    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_errors<DD>, dim3(kErrBlocks), dim3(256), 0, st, g.dev));
-    hipLaunchKernelGGL(k_err_reduce, dim3(1), dim3(1024), 0, st, g.dev, kErrBlocks);
+    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_errors<DD>, dim3(g.err_blocks), dim3(256), 0, st, g.dev));
+    hipLaunchKernelGGL(k_err_reduce, dim3(1), dim3(1024), 0, st, g.dev, g.err_blocks);
     return hipGetLastError();
 }
 
 hipError_t launch_errors_stale(Grid& g, hipStream_t st) {
-    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_errors_stale<DD>, dim3(kErrBlocks), dim3(256), 0, st, g.dev));
-    hipLaunchKernelGGL(k_err_reduce, dim3(1), dim3(1024), 0, st, g.dev, kErrBlocks);
+    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_errors_stale<DD>, dim3(g.err_blocks), dim3(256), 0, st, g.dev));
+    hipLaunchKernelGGL(k_err_reduce, dim3(1), dim3(1024), 0, st, g.dev, g.err_blocks);
     return hipGetLastError();
 }
 
 hipError_t launch_select(Grid& g, hipStream_t st) {
-    hipLaunchKernelGGL(k_select, dim3(kChunkBlocks), dim3(256), 0, st, g.dev);
+    hipLaunchKernelGGL(k_select, dim3(g.chunk_blocks), dim3(256), 0, st, g.dev);
     return hipGetLastError();
 }
 
 hipError_t launch_scan(Grid& g, hipStream_t st) {
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, g.dev, kChunkBlocks);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, g.dev, g.chunk_blocks);
     return hipGetLastError();
 }
 
 hipError_t launch_place(Grid& g, hipStream_t st) {
-    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_place<DD>, dim3(kChunkBlocks), dim3(256), 0, st, g.dev));
+    hipLaunchKernelGGL(k_compact, dim3(g.chunk_blocks), dim3(256), 0, st, g.dev);
+    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_place<DD>, dim3(kPlaceBlocks), dim3(256), 0, st, g.dev));
     return hipGetLastError();
 }
 
@@ -749,11 +758,12 @@ hipError_t launch_initial_b(int D, const DevGrid* grids, int n, int blocks, cons
 
 hipError_t launch_pass_b(int D, const DevGrid* grids, int n, int blocks, hipStream_t st) {
     hipLaunchKernelGGL(k_pass_begin_b, dim3((n + 63) / 64), dim3(64), 0, st, grids, n);
-    const int eb = blocks < kErrBlocks ? blocks : kErrBlocks;
+    const int eb = blocks < 1024 ? blocks : 1024;
     MDAS_DISPATCH_D(D, hipLaunchKernelGGL(k_errors_stale_b<DD>, dim3(eb, n), dim3(256), 0, st, grids));
     hipLaunchKernelGGL(k_err_reduce_b, dim3(n), dim3(1024), 0, st, grids, eb);
     hipLaunchKernelGGL(k_select_b, dim3(blocks, n), dim3(256), 0, st, grids);
     hipLaunchKernelGGL(k_scan_counts_b, dim3(n), dim3(1024), 0, st, grids, blocks);
+    hipLaunchKernelGGL(k_compact_b, dim3(blocks, n), dim3(256), 0, st, grids);
     MDAS_DISPATCH_D(D, hipLaunchKernelGGL(k_place_b<DD>, dim3(blocks, n), dim3(256), 0, st, grids));
     return hipGetLastError();
 }

@@ -41,9 +41,24 @@ hipError_t dalloc(T** p, size_t n) {
     return hipMalloc((void**)p, sizeof(T) * (n ? n : 1));
 }
 
+// The LCG x -> 1664525 x + 1013904223 (mod 2^32) applied k times is
+// x -> A x + C; A is odd, so the map is invertible: x = A^-1 (y - C), with
+// A^-1 by Newton's iteration mod 2^32.
+void lcg_jump(unsigned k, unsigned& A, unsigned& C, unsigned& Ainv) {
+    A = 1u;
+    C = 0u;
+    for (unsigned j = 0; j < k; ++j) {
+        A = 1664525u * A;
+        C = 1664525u * C + 1013904223u;
+    }
+    unsigned x = A;  // correct to 3 bits for odd A
+    for (int it = 0; it < 5; ++it) x *= 2u - A * x;
+    Ainv = x;
+}
+
 void free_grid(Grid& g) {
     mdas::DevGrid& d = g.dev;
-    void* ptrs[] = {d.coords,   d.values, d.leaf,     d.err,  d.seed,       d.flags, d.pts,
+    void* ptrs[] = {d.coords,   d.values, d.leaf,     d.err,  d.seed,       d.flags, d.pts, d.qlist,
                     d.split_list, d.part, d.part_off, d.epart, d.rrec,      d.cnt,
                     d.pstats,   g.keys,   g.keys_alt, g.tile_start, g.tile_end, g.rs_hist,
                     g.d_scalar, g.learn.feat, g.learn.perr, g.learn.rows};
@@ -164,6 +179,7 @@ int mdas_create(const mdas_config* cfg, mdas_handle* out) {
     if (e == hipSuccess) e = dalloc(&d.values, cs * 3);
     if (e == hipSuccess) e = dalloc(&d.leaf, cl * mdas::leaf_quads(D));
     if (e == hipSuccess) e = dalloc(&d.pts, cl * 4 * D);
+    if (e == hipSuccess) e = dalloc(&d.qlist, cl);
     if (e == hipSuccess) e = dalloc(&d.err, cl);
     if (e == hipSuccess) e = dalloc(&d.seed, cl);
     if (e == hipSuccess) e = dalloc(&d.flags, cl);
@@ -191,6 +207,7 @@ int mdas_create(const mdas_config* cfg, mdas_handle* out) {
     d.cap_s = g.cap_s;
     d.cap_l = g.cap_l;
     d.cand = g.cand;
+    lcg_jump((unsigned)(g.cand * g.D), d.jmul, d.jadd, d.jinv);
     d.b = g.b;
     d.e = g.e;
     d.alpha = g.alpha;
@@ -737,19 +754,20 @@ static double kernel_bytes(const Grid& g, const std::string& k) {
         else if (k == "k_select")  // error, state, flags read; state, flags written
             b += 14.0 * p.L;
         else if (k == "k_scan_counts")
-            b += (16.0 + 8.0) * mdas::kChunkBlocks;
-        else if (k == "k_place")
-            b += 1.0 * p.L +                                              // flags
-                 (double)p.sel * (4.0 + LB + 4.0 * D + 4.0 + 1.0) +       // state, record, new sample
+            b += (16.0 + 8.0) * g.chunk_blocks;
+        else if (k == "k_place")  // compaction + list placement
+            b += 2.0 * p.L +                                              // flags read, count written
+                 (double)p.sel * (16.0 + 16.0 + 4.0 + LB + 4.0 * D) +     // list written + read, state,
+                                                                          // record, new sample
                  4.0 * D * p.selcnt +                                     // the leaf's points
                  (double)(p.sel - p.split) * (16.0 + 4.0 * D) +           // slots + point of an insert
-                 (double)p.split * (2.0 * LB + 20.0 * D + 8.0 + 4.0 + 2.0 + 4.0);  // two records
+                 (double)p.split * (2.0 * LB + 20.0 * D + 8.0 + 4.0 + 1.0 + 4.0);  // two records
                                                                           // and their points, errors
     }
     const double L = (double)g.host.n_leaves, N = (double)g.host.n_samples;
     const double K = (double)g.last_n_keys;
     if (k == "k_recon_prep") return L * (LB + 48.0) + 12.0 * N;
-    if (k == "k_scan_keys") return (16.0 + 8.0) * mdas::kChunkBlocks;
+    if (k == "k_scan_keys") return (16.0 + 8.0) * g.chunk_blocks;
     if (k == "k_recon_emit") return L * 16.0 + K * 8.0;
     if (k == "k_rs_hist") return K * 8.0 * g.recon_sort_passes;
     if (k == "k_rs_scan") return 3.0 * 4.0 * 256 * 1024 * g.recon_sort_passes;
@@ -843,6 +861,17 @@ int mdas_autotune_recon(mdas_handle h, float* pixels, int w, int hgt, void* stre
     g.timing = timing;
     if (best_tw) *best_tw = btw;
     if (best_th) *best_th = bth;
+    return MDAS_OK;
+}
+
+int mdas_set_launch_shape(mdas_handle h, int chunk_blocks, int err_blocks) {
+    if (!h) return fail(MDAS_ERR_INVALID, "null handle");
+    if (chunk_blocks < 64 || chunk_blocks > mdas::kChunkBlocks || err_blocks < 64 ||
+        err_blocks > mdas::kErrBlocks)
+        return fail(MDAS_ERR_INVALID, "launch shape out of range");
+    HIP_TRY(hipDeviceSynchronize());
+    h->g.chunk_blocks = chunk_blocks;
+    h->g.err_blocks = err_blocks;
     return MDAS_OK;
 }
 

@@ -43,8 +43,9 @@ struct KRec {
     hipEvent_t a, b;
 };
 
-static const int kChunkBlocks = 2048;   // workgroups of the chunked leaf kernels
-static const int kErrBlocks = 1024;     // workgroups of the error kernels
+static const int kChunkBlocks = 32768;  // max workgroups of the chunked leaf kernels
+static const int kErrBlocks = 8192;     // max workgroups of the error kernels
+static const int kPlaceBlocks = 4096;   // workgroups of the list placement
 static const int kMaxPassStats = 4096;
 
 // Float4 quads of one leaf record: the int4 of sample slots, then the box
@@ -59,6 +60,7 @@ struct DevGrid {
     float* values;         // [cap_s * 3]  float3 AoS
     float4* leaf;          // [cap_l * leaf_quads(D)]  slots (~D = empty) + box
     float* pts;            // [cap_l * 4 * D]  copy of each leaf's sample points by slot
+    int4* qlist;           // [cap_l]      placement list (leaf, sample, right child, count)
     float* err;            // [cap_l]      < 0 = stale
     unsigned* seed;        // [cap_l]      LCG state, 0 = unset
     unsigned char* flags;  // [cap_l]      bits 0-2 count, 3 selected, 4 split
@@ -73,6 +75,7 @@ struct DevGrid {
     int D;
     int cap_s, cap_l;
     int cand;
+    unsigned jmul, jadd, jinv;  // LCG jump over candidatesNum * D draws and its inverse
     int b, e;
     float alpha, sX, sY;
 };
@@ -145,6 +148,9 @@ struct Grid {
     std::vector<PassStat> pstats_host;
     long long recon_pixels = 0;
     int recon_sort_passes = 0;
+    // launch shape of the per-pass kernels (mdas_set_launch_shape)
+    int chunk_blocks = 4096;   // leaf chunks of selection / compaction / reconstruction
+    int err_blocks = 1024;     // workgroups of the error pass
     // learned proposal (mode bits: 1 collect rows, 2 score with the MLP)
     int learn_mode = 0;
     LearnDev learn{};

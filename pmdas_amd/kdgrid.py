@@ -234,6 +234,9 @@ class KDGrid:
                                                   max_rows))
         return out[:n]
 
+    def set_launch_shape(self, chunk_blocks, err_blocks):
+        check(_lib.load().mdas_set_launch_shape(self._h, chunk_blocks, err_blocks))
+
     def set_recon_tile(self, tw, th):
         check(_lib.load().mdas_set_recon_tile(self._h, tw, th))
 
