@@ -209,6 +209,89 @@ def learned_comparison(log=print):
             "seconds": round(time.perf_counter() - t0, 1)}
 
 
+FRAMES = dict(dim=6, kind=4, W=3840, H=2160, spp=64, b=1, alpha=0.25, cand=4, frames=16,
+              first_seed=100, train_seeds=100, train_W=192, train_H=108,
+              val_seeds=(1000, 1001), val_W=240, val_H=135, kappas=(1.0, 0.5, 0.25, 0.125))
+
+
+def run_frames(log=print):
+    """6D ambient occlusion at 3840x2160, 64 spp, 16 frames with per-frame
+    seeds, learned proposal (network trained on plain runs of 100 small AO
+    scenes, score exponent chosen on 2 validation scenes) and batching: the
+    frames stream through a batch of as many frames as device memory holds
+    (each frame's grid needs ~130 GB at this size, so 16 at once cannot be
+    resident); each batch runs as one captured hipGraph per block of passes,
+    with the network's inference fused into the error pass."""
+    from pmdas_amd import hip, kdgrid, learned as L
+    c = FRAMES
+    t0 = time.perf_counter()
+    X, Y, n = L.collect(range(c["train_seeds"]), c["train_W"], c["train_H"], c["spp"], dim=c["dim"],
+                        kind=c["kind"], cap_rows=1 << 25, log=log)
+    net, losses = L.train(X, Y, n, 4, 65536, 1e-3, seed=0, log=log)
+    X.free()
+    Y.free()
+    kappa, _ = L.tune_kappa(net, c["val_seeds"], c["val_W"], c["val_H"], c["spp"], c["kappas"],
+                            dim=c["dim"], kind=c["kind"], log=log)
+    t_train = time.perf_counter() - t0
+    W, H, D = c["W"], c["H"], c["dim"]
+    e = L.extra_bits(W, H, D, c["b"])
+    target = c["spp"] * W * H
+    cap = int(target * 1.25) + 4 * (1 << (c["b"] * D + 2 * e)) // 8
+    per_frame = cap * (4 * D + 12) + (cap // 2) * (16 * 4 + 16 * D + 4 + 4 + 1 + 4 + 16 + 48 + 1 + 68)
+    free, _ = hip.mem_info()
+    F = int(max(1, min(c["frames"], (free * 0.85) // per_frame)))
+    log(f"[bench] frames: {per_frame / 2**30:.0f} GiB per frame grid, {free / 2**30:.0f} GiB free, "
+        f"{F} frame(s) per batch")
+    grids = [kdgrid.KDGrid(D, cap, c["cand"], c["b"], e, c["alpha"], float(W), float(H))
+             for _ in range(F)]
+    for g in grids:
+        g.set_learned(2, net)
+        g.set_learned_exponent(kappa)
+    px = hip.DeviceBuffer(W * H * 16)
+    samples, wall = 0, 0.0
+    tile = None
+    for f0 in range(0, c["frames"], F):
+        fr = list(range(f0, min(c["frames"], f0 + F)))
+        scenes = [kdgrid.Scene(c["kind"], c["first_seed"] + f, D, float(W), float(H)) for f in fr]
+        batch = kdgrid.Batch(grids[:len(fr)], scenes)
+        hip.synchronize()
+        t1 = time.perf_counter()
+        batch.run([target] * len(fr), use_graph=True)
+        for g in grids[:len(fr)]:
+            if tile is None:
+                tile = g.autotune_recon(px.ptr, W, H)
+            g.Integrate(px.ptr, W, H)
+            samples += g.GetNumberOfSamples()
+        hip.synchronize()
+        wall += time.perf_counter() - t1
+        log(f"[bench] frames {fr[0]}..{fr[-1]}: {samples} samples so far, {wall:.2f} s")
+        batch.close()
+        for sc in scenes:
+            sc.close()
+    # MDAS-phase throughput of one frame (same grid, learned, timed per kernel)
+    g = grids[0]
+    sc = kdgrid.Scene(c["kind"], c["first_seed"], D, float(W), float(H))
+    g.set_timing(True)
+    g.Run(sc, target)
+    g.Integrate(px.ptr, W, H)
+    pt = g.phase_times()
+    mdas_ms = pt["initial_ms"] + pt["errors_ms"] + pt["adaptive_ms"] + pt["recon_ms"]
+    n1 = g.GetNumberOfSamples()
+    sc.close()
+    px.free()
+    for g in grids:
+        g.close()
+    net.close()
+    return {"workload": "mdas-6d-ao-4k-64spp-16frames-learned", "frames": c["frames"],
+            "frames_per_batch": F, "samples": samples,
+            "e2e_msamples_per_s": round(samples / wall / 1e6, 1),
+            "note": "wall time of all frames: batched sampling with the integrand + reconstruction",
+            "one_frame_mdas_ms": round(mdas_ms, 3),
+            "one_frame_msamples_per_s": round(n1 / (mdas_ms * 1e3), 1),
+            "one_frame_integrand_ms": round(pt["integrand_ms"], 3), "kappa": kappa,
+            "train_seconds": round(t_train, 1), "recon_tile": tile}
+
+
 def roofline(res, traffic_file):
     """Dominant MDAS kernel (integrand excluded): algorithmic bytes / its time."""
     ks = {k: v for k, v in res["kernels"].items() if k != "k_eval"}
@@ -283,6 +366,7 @@ def main():
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-frames", action="store_true")
     ap.add_argument("--cpu-shrink", type=int, default=0,
                     help="run the CPU baseline on a 2^-k per-axis version of the workload")
     a = ap.parse_args()
@@ -343,6 +427,8 @@ def main():
         line["batch"] = run_batch(max(1, min(2, a.steps)), log)
     if not a.no_learned:
         line["learned"] = learned_comparison(log)
+    if not a.no_frames:
+        line["frames"] = run_frames(log)
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_shrink, log)
     else:

@@ -188,7 +188,10 @@ int mdas_get_recon_keys(mdas_handle h, uint64_t* out, long long capacity, long l
  * together: one launch per kernel and pass for every image of a dimension,
  * the block of passes captured once into a hipGraph and replayed until every
  * image meets its stopping rule.  Each image's result equals mdas_run on it
- * alone.  The grids and scenes must outlive the batch. */
+ * alone.  The grids and scenes must outlive the batch.  A grid's learned
+ * mode (mdas_set_learned / _exponent) is taken at mdas_batch_create: the
+ * batched error pass then computes features and network scores for the
+ * learned members and the plain error for the others. */
 typedef struct mdas_batch* mdas_batch_handle;
 int mdas_batch_create(mdas_handle* grids, mdas_scene_handle* scenes, int n, mdas_batch_handle* out);
 int mdas_batch_destroy(mdas_batch_handle b);

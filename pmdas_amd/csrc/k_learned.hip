@@ -113,7 +113,8 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
     const int L_old = a.cnt->n_leaves_old;
     const long long rows_base = (first || l.X == nullptr) ? 0 : *l.rows;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const bool model = l.mlp.p != nullptr;
+    const bool feats = l.feat != nullptr;  // false: a plain member of a batch
+    const bool model = feats && l.mlp.p != nullptr;
     float m = 0.0f;
     int n_stale = 0, n_vals = 0;
     for (int t0 = blk * 256; t0 < n_items; t0 += nblk * 256) {
@@ -126,7 +127,7 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
         }
         float depth = 0.0f, pe = 0.0f;
         float pf[kFeat];
-        if (act && !first) {
+        if (act && !first && feats) {
 #pragma unroll
             for (int k = 0; k < kFeat; ++k) pf[k] = l.feat[(size_t)c[0] * kFeat + k];
             depth = pf[11] + 1.0f;
@@ -150,7 +151,7 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
                 for (int k = 0; k < kFeat; ++k) f[h][k] = 0.0f;
             }
         }
-        if (act && !first && l.X != nullptr) {
+        if (act && !first && feats && l.X != nullptr) {
             const long long row = rows_base + t;
             if (row < l.cap_rows) {
 #pragma unroll
@@ -165,7 +166,7 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            if (c[h] >= 0) {
+            if (c[h] >= 0 && feats) {
                 float4* fo = (float4*)(l.feat + (size_t)c[h] * kFeat);
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
@@ -233,10 +234,14 @@ template <int D>
 __global__ __launch_bounds__(256) void k_errors_learned(DevGrid a, LearnDev l) {
     errors_learned_body<D>(a, l, blockIdx.x, gridDim.x);
 }
+template <int D>
+__global__ __launch_bounds__(256) void k_errors_learned_b(const DevGrid* grids, const LearnDev* ls) {
+    errors_learned_body<D>(grids[blockIdx.y], ls[blockIdx.y], blockIdx.x, gridDim.x);
+}
 
 // Folds the partials like the plain pass, then advances the row counter by
 // the splits whose rows this pass offered.
-__global__ __launch_bounds__(1024) void k_err_reduce_learned(DevGrid a, LearnDev l, int G) {
+__device__ __forceinline__ void err_reduce_learned_body(const DevGrid& a, const LearnDev& l, int G) {
     __shared__ unsigned sm[16];
     __shared__ int ss[16], sv[16];
     if (a.cnt->done) return;
@@ -271,8 +276,15 @@ __global__ __launch_bounds__(1024) void k_err_reduce_learned(DevGrid a, LearnDev
         a.cnt->gmax_bits = mm;
         a.cnt->stale += s1;
         a.cnt->stale_vals += s2;
-        if (l.X != nullptr && a.cnt->passes > 0) *l.rows += a.cnt->new_nodes;
+        if (l.X != nullptr && l.feat != nullptr && a.cnt->passes > 0) *l.rows += a.cnt->new_nodes;
     }
+}
+__global__ __launch_bounds__(1024) void k_err_reduce_learned(DevGrid a, LearnDev l, int G) {
+    err_reduce_learned_body(a, l, G);
+}
+__global__ __launch_bounds__(1024) void k_err_reduce_learned_b(const DevGrid* grids,
+                                                                const LearnDev* ls, int G) {
+    err_reduce_learned_body(grids[blockIdx.x], ls[blockIdx.x], G);
 }
 
 hipError_t launch_errors_learned(Grid& g, hipStream_t st) {
@@ -284,6 +296,20 @@ hipError_t launch_errors_learned(Grid& g, hipStream_t st) {
     default: return hipErrorInvalidValue;
     }
     hipLaunchKernelGGL(k_err_reduce_learned, dim3(1), dim3(1024), 0, st, g.dev, g.learn, g.err_blocks);
+    return hipGetLastError();
+}
+
+hipError_t launch_errors_learned_b(int D, const DevGrid* grids, const LearnDev* ls, int n, int blocks,
+                                   hipStream_t st) {
+    const int eb = blocks < 1024 ? blocks : 1024;
+    switch (D) {
+    case 3: hipLaunchKernelGGL(k_errors_learned_b<3>, dim3(eb, n), dim3(256), 0, st, grids, ls); break;
+    case 4: hipLaunchKernelGGL(k_errors_learned_b<4>, dim3(eb, n), dim3(256), 0, st, grids, ls); break;
+    case 5: hipLaunchKernelGGL(k_errors_learned_b<5>, dim3(eb, n), dim3(256), 0, st, grids, ls); break;
+    case 6: hipLaunchKernelGGL(k_errors_learned_b<6>, dim3(eb, n), dim3(256), 0, st, grids, ls); break;
+    default: return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(k_err_reduce_learned_b, dim3(n), dim3(1024), 0, st, grids, ls, eb);
     return hipGetLastError();
 }
 

@@ -756,11 +756,17 @@ hipError_t launch_initial_b(int D, const DevGrid* grids, int n, int blocks, cons
     return hipGetLastError();
 }
 
-hipError_t launch_pass_b(int D, const DevGrid* grids, int n, int blocks, hipStream_t st) {
+hipError_t launch_pass_b(int D, const DevGrid* grids, int n, int blocks, hipStream_t st,
+                         const LearnDev* learns) {
     hipLaunchKernelGGL(k_pass_begin_b, dim3((n + 63) / 64), dim3(64), 0, st, grids, n);
     const int eb = blocks < 1024 ? blocks : 1024;
-    MDAS_DISPATCH_D(D, hipLaunchKernelGGL(k_errors_stale_b<DD>, dim3(eb, n), dim3(256), 0, st, grids));
-    hipLaunchKernelGGL(k_err_reduce_b, dim3(n), dim3(1024), 0, st, grids, eb);
+    if (learns) {
+        const hipError_t e = launch_errors_learned_b(D, grids, learns, n, blocks, st);
+        if (e != hipSuccess) return e;
+    } else {
+        MDAS_DISPATCH_D(D, hipLaunchKernelGGL(k_errors_stale_b<DD>, dim3(eb, n), dim3(256), 0, st, grids));
+        hipLaunchKernelGGL(k_err_reduce_b, dim3(n), dim3(1024), 0, st, grids, eb);
+    }
     hipLaunchKernelGGL(k_select_b, dim3(blocks, n), dim3(256), 0, st, grids);
     hipLaunchKernelGGL(k_scan_counts_b, dim3(n), dim3(1024), 0, st, grids, blocks);
     hipLaunchKernelGGL(k_compact_b, dim3(blocks, n), dim3(256), 0, st, grids);

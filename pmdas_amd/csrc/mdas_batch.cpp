@@ -24,6 +24,7 @@ struct Group {
     const void** d_params = nullptr;     // device array of scene parameter blocks
     int* d_limits = nullptr;
     int* d_status = nullptr;             // (done, status) per member
+    mdas::LearnDev* d_learn = nullptr;   // per member, when any member is in learned mode
     int blocks = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -82,9 +83,13 @@ int mdas_batch_create(mdas_handle* grids, mdas_scene_handle* scenes, int n, mdas
         gr.blocks = std::max(64, std::min(mdas::kChunkBlocks, 8192 / m));
         std::vector<mdas::DevGrid> dg(m);
         std::vector<const void*> dp(m);
+        std::vector<mdas::LearnDev> dl(m);
+        bool learned = false;
         for (int k = 0; k < m; ++k) {
             dg[k] = grids[gr.members[k]]->g.dev;
             dp[k] = scenes[gr.members[k]]->dev;
+            dl[k] = grids[gr.members[k]]->g.learn;  // feat == null: plain error for this member
+            learned = learned || grids[gr.members[k]]->g.learn_mode != 0;
         }
         hipError_t e = hipMalloc(&gr.d_grids, sizeof(mdas::DevGrid) * m);
         if (e == hipSuccess) e = hipMalloc((void**)&gr.d_params, sizeof(void*) * m);
@@ -92,6 +97,9 @@ int mdas_batch_create(mdas_handle* grids, mdas_scene_handle* scenes, int n, mdas
         if (e == hipSuccess) e = hipMalloc(&gr.d_status, sizeof(int) * 2 * m);
         if (e == hipSuccess) e = hipMemcpy(gr.d_grids, dg.data(), sizeof(mdas::DevGrid) * m, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(gr.d_params, dp.data(), sizeof(void*) * m, hipMemcpyHostToDevice);
+        if (e == hipSuccess && learned) e = hipMalloc(&gr.d_learn, sizeof(mdas::LearnDev) * m);
+        if (e == hipSuccess && learned)
+            e = hipMemcpy(gr.d_learn, dl.data(), sizeof(mdas::LearnDev) * m, hipMemcpyHostToDevice);
         b->groups.push_back(gr);
         if (e != hipSuccess) {
             mdas_batch_destroy(b);
@@ -112,6 +120,7 @@ int mdas_batch_destroy(mdas_batch_handle b) {
         if (g.d_params) (void)hipFree((void*)g.d_params);
         if (g.d_limits) (void)hipFree(g.d_limits);
         if (g.d_status) (void)hipFree(g.d_status);
+        if (g.d_learn) (void)hipFree(g.d_learn);
     }
     if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
     delete b;
@@ -161,7 +170,7 @@ int mdas_batch_run(mdas_batch_handle b, const long long* targets, void* stream, 
                     B_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
                     hipError_t ce = hipSuccess;
                     for (int p = 0; p < B && ce == hipSuccess; ++p) {
-                        ce = mdas::launch_pass_b(g.D, g.d_grids, m, g.blocks, st);
+                        ce = mdas::launch_pass_b(g.D, g.d_grids, m, g.blocks, st, g.d_learn);
                         if (ce == hipSuccess) ce = mdas::launch_eval_b(g.d_grids, g.d_params, m, g.blocks, st);
                     }
                     B_TRY(hipStreamEndCapture(st, &g.graph));
@@ -171,7 +180,7 @@ int mdas_batch_run(mdas_batch_handle b, const long long* targets, void* stream, 
                 B_TRY(hipGraphLaunch(g.exec, st));
             } else {
                 for (int p = 0; p < B; ++p) {
-                    B_TRY(mdas::launch_pass_b(g.D, g.d_grids, m, g.blocks, st));
+                    B_TRY(mdas::launch_pass_b(g.D, g.d_grids, m, g.blocks, st, g.d_learn));
                     B_TRY(mdas::launch_eval_b(g.d_grids, g.d_params, m, g.blocks, st));
                 }
             }

@@ -195,7 +195,9 @@ hipError_t launch_fill_f32(float* p, float v, long long n, hipStream_t st);
 // Batched forms over a device array of n grids of dimension D (k_sampling.hip)
 hipError_t launch_initial_b(int D, const DevGrid* grids, int n, int blocks, const int* limits,
                             hipStream_t st);
-hipError_t launch_pass_b(int D, const DevGrid* grids, int n, int blocks, hipStream_t st);
+// learns: device array of n LearnDev (the learned error pass) or null (plain)
+hipError_t launch_pass_b(int D, const DevGrid* grids, int n, int blocks, hipStream_t st,
+                         const LearnDev* learns = nullptr);
 
 // Integrand (k_integrand.hip)
 hipError_t launch_eval(Grid& g, const void* dparams, int first, int count, bool dev_range,
@@ -216,6 +218,8 @@ hipError_t launch_bytes(const float* pixels, unsigned char* out, long long n, hi
 
 // learned proposal (k_learned.hip)
 hipError_t launch_errors_learned(Grid& g, hipStream_t st);
+hipError_t launch_errors_learned_b(int D, const DevGrid* grids, const LearnDev* ls, int n, int blocks,
+                                   hipStream_t st);
 hipError_t launch_mlp_pack(float* p, unsigned short* w1p, unsigned short* w2p, float* w2r,
                            float* w3r, hipStream_t st);
 hipError_t launch_mlp_forward(const MlpDev& M, const float* X, long long n, float* Y,

@@ -41,6 +41,13 @@ def synchronize():
     _chk(lib().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
 
+def mem_info():
+    """(free, total) bytes of device memory."""
+    free, total = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    _chk(lib().hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)), "hipMemGetInfo")
+    return free.value, total.value
+
+
 class DeviceBuffer:
     """An owned hipMalloc allocation."""
 

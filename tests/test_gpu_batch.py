@@ -79,3 +79,46 @@ def test_batch_rejects_mismatched_scene(kd):
     sc4 = kd.Scene(2, 0, 4, 64.0, 48.0)
     with pytest.raises(MdasError):
         kd.Batch([g], [sc4])
+
+
+def test_batch_learned_equals_single_runs(kd):
+    """Learned members (MLP scoring fused into the batched error pass) next to
+    a plain member of the same dimension: every image equals its solo run with
+    the same mode, bit for bit, with graph replay."""
+    from oracle import mlp_oracle as mo
+    r = np.random.default_rng(9)
+    p = (r.standard_normal(mo.N_FLOATS) * 0.15).astype(np.float32)
+    p[mo.OFF_MU:mo.OFF_ISD] = 0.0
+    p[mo.OFF_ISD:] = 0.2
+    net = kd.Mlp(0)
+    net.set_params(p)
+    specs = IMAGES[:3] + [IMAGES[5]]
+    learned = [True, False, True, True]
+    made = [_make(kd, s) for s in specs]
+    for (g, *_), on in zip(made, learned):
+        if on:
+            g.set_learned(2, net)
+            g.set_learned_exponent(0.5)
+    batch = kd.Batch([m[0] for m in made], [m[1] for m in made])
+    targets = [m[2] for m in made]
+    for replay in range(2):
+        print(f"learned batch replay {replay}", flush=True)
+        batch.run(targets, use_graph=True)
+        for (g, sc, target, args), on in zip(made, learned):
+            st = g.download_state()
+            solo = kd.KDGrid(*args)
+            if on:
+                solo.set_learned(2, net)
+                solo.set_learned_exponent(0.5)
+            solo.Run(sc, target)
+            ref = solo.download_state()
+            for k in st:
+                assert np.array_equal(st[k], ref[k]), (replay, k, args, on)
+            if on:  # leaves split by the last pass have no features yet (err < 0)
+                live = st["err"] >= 0
+                fa, ea = g.leaf_features()
+                fb, eb = solo.leaf_features()
+                assert np.array_equal(fa[live], fb[live]) and np.array_equal(ea[live], eb[live])
+            solo.close()
+    batch.close()
+    net.close()
