@@ -76,7 +76,8 @@ int mdas_integrate(mdas_handle h, float* pixels, int width, int height, void* st
 /* Reference-style reconstruction with float atomics (kdgrid.cu:483-582);
  * same result up to summation order.  Kept for comparison. */
 int mdas_integrate_scatter(mdas_handle h, float* pixels, int width, int height, void* stream);
-/* KDGrid::ConvertToBytes (kdgrid.h:61; kdgrid.cu:839-846) -> uchar4 sRGB */
+/* KDGrid::ConvertToBytes (kdgrid.h:61; kdgrid.cu:839-846) -> uchar4 sRGB.
+ * Uses no grid state: h may be NULL (e.g. for a plain QMC image). */
 int mdas_convert_to_bytes(mdas_handle h, const float* pixels, unsigned char* bytes, int width,
                           int height, void* stream);
 /* KDGrid::SamplingDensity (kdgrid.h:60; kdgrid.cu:822-837): host float4 image */

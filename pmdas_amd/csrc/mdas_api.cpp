@@ -96,11 +96,14 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 }
 
 // Fold the recorded kernel events into per-kernel totals and phase times.
-void collect_kernel_times(Grid& g) {
+// Folds the timed launches into per-kernel totals; seq (optional) receives
+// every launch's (name, ms) in launch order.
+void collect_kernel_times(Grid& g, std::vector<std::pair<std::string, double>>* seq = nullptr) {
     if (g.krecs.empty()) return;
     if (g.krecs.back().b) (void)hipEventSynchronize(g.krecs.back().b);
     for (const auto& r : g.krecs) {
         const double ms = (r.a && r.b) ? elapsed(r.a, r.b) : 0.0;
+        if (seq) seq->emplace_back(r.name, ms);
         bool found = false;
         for (auto& k : g.kstats)
             if (k.name == r.name) {
@@ -658,6 +661,15 @@ int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream)
     const int L0 = 1 << (g.b * g.D + 2 * g.e);
     const long long limit = std::min<long long>(target, g.max_samples);
     const int B = std::max(1, g.passes_per_sync);
+    // the log needs per-pass times: logging times the run like mdas_set_timing
+    const bool logging = !g.log_path.empty();
+    const bool timing_was = g.timing;
+    if (logging) g.timing = true;
+    struct Restore {
+        Grid& g;
+        bool v;
+        ~Restore() { g.timing = v; }
+    } restore{g, timing_was};
     g.kstats.clear();
     g.krecs.clear();
     g.evnext = 0;
@@ -700,18 +712,27 @@ int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream)
         if (enqueued >= 20000) return fail(MDAS_ERR_STATE, "sampling loop exceeded 20000 passes");
     }
     g.passes = g.host.passes;
-    collect_kernel_times(g);
+    std::vector<std::pair<std::string, double>> seq;
+    collect_kernel_times(g, logging ? &seq : nullptr);
     g.pstats_host.assign((size_t)std::min(g.passes, mdas::kMaxPassStats), mdas::PassStat{});
     if (!g.pstats_host.empty())
         HIP_TRY(hipMemcpy(g.pstats_host.data(), g.dev.pstats, sizeof(mdas::PassStat) * g.pstats_host.size(),
                           hipMemcpyDeviceToHost));
-    if (!g.log_path.empty()) {
+    if (logging && seq.size() >= 2) {
+        // the app's and the library's keys in the order they write them
+        // (kdgrid.cu:670-671, 713, 769-770; optixMotionBlur.cpp:774, 788):
+        // the initial grid, its integrand launch, then per pass the error
+        // estimate, the new samples, the adaptive step and the integrand
         log_kv(g, "INITIAL SAMPLES", 4.0 * L0);
-        log_kv(g, "INITIAL SAMPLING TIME", g.t_initial);
-        for (auto& p : g.pstats_host) log_kv(g, "SAMPLES", p.sel);
-        log_kv(g, "COMPUTE ERRORS TIME", g.t_errors);
-        log_kv(g, "ADAPTIVE SAMPLING TIME", g.t_adaptive);
-        log_kv(g, "TRACE TIME", g.t_integrand);
+        log_kv(g, "INITIAL SAMPLING TIME", seq[0].second);
+        log_kv(g, "TRACE TIME", seq[1].second);
+        size_t k = 2;
+        for (int p = 0; p < g.passes && k + 4 < seq.size(); ++p, k += 5) {
+            log_kv(g, "COMPUTE ERRORS TIME", seq[k].second);
+            log_kv(g, "SAMPLES", p < (int)g.pstats_host.size() ? g.pstats_host[p].sel : 0);
+            log_kv(g, "ADAPTIVE SAMPLING TIME", seq[k + 1].second + seq[k + 2].second + seq[k + 3].second);
+            log_kv(g, "TRACE TIME", seq[k + 4].second);
+        }
     }
     if (g.host.status) {
         int s2 = g.host.status;

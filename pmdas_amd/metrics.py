@@ -1,8 +1,9 @@
 """Image error metrics and image files.
 
 mse / rel_mse follow the reference's experiment scripts
-(SDK/data/Scripts/parse_logs.py:43-50): rel_mse = mean((ref - img)^2 /
-(ref^2 + 1e-2)) with NaN -> 0, over all pixels and the three colour channels.
+(SDK/data/Scripts/parse_logs.py:43-50): NaN pixels of the image count as 0,
+mse = mean((ref - img)^2), rel_mse = mean((ref - img)^2 / (ref^2 + 1e-2)),
+over all pixels and the three colour channels.
 Writers: binary PPM (8-bit sRGB), PFM (float RGB) and uncompressed OpenEXR
 (float RGBA scanlines) -- the reference writes EXR/PNG through tinyexr
 (sutil.cpp:556-651); these are minimal writers of the published formats.
@@ -12,18 +13,20 @@ import struct
 import numpy as np
 
 
+def _rgb(img):
+    a = np.array(np.asarray(img)[..., :3], np.float64)
+    a[np.isnan(a)] = 0.0
+    return a
+
+
 def mse(img, ref):
-    a = np.asarray(img, np.float64)[..., :3]
-    b = np.asarray(ref, np.float64)[..., :3]
+    a, b = _rgb(img), np.asarray(ref, np.float64)[..., :3]
     return float(np.mean((b - a) ** 2))
 
 
 def rel_mse(img, ref, eps=1e-2):
-    a = np.asarray(img, np.float64)[..., :3]
-    b = np.asarray(ref, np.float64)[..., :3]
-    r = (b - a) ** 2 / (b * b + eps)
-    r[np.isnan(r)] = 0.0
-    return float(np.mean(r))
+    a, b = _rgb(img), np.asarray(ref, np.float64)[..., :3]
+    return float(np.mean((b - a) ** 2 / (b * b + eps)))
 
 
 def write_ppm(path, rgba8):

@@ -280,3 +280,32 @@ def test_render_qmc_bit_exact(mdas, kind, dim, fp64):
     want = pyoracle.qmc_image(sc.params_bytes(), W, H, 12, 3, 2, fp64=fp64)
     u = np.uint64 if fp64 else np.uint32
     assert np.array_equal(got.view(u), want.view(u))
+
+
+def _log_values(path, key):
+    """All values logged under `key` (the KEY / value line pairs the
+    reference's parse_logs.get_values reads)."""
+    lines = open(path).read().split("\n")
+    return [float(lines[i + 1]) for i in range(len(lines) - 1) if lines[i] == key]
+
+
+def test_log_keys_per_pass(mdas, tmp_path):
+    """mdas_run + mdas_integrate with a log: the reference's keys, one
+    COMPUTE ERRORS / SAMPLES / ADAPTIVE SAMPLING / TRACE entry per pass."""
+    log = str(tmp_path / "run.log")
+    W, H, spp = 64, 48, 12
+    target = spp * W * H
+    g = mdas.KDGrid(5, int(target * 1.6) + (4 << 9), 4, 1, 2, 0.25, float(W), float(H), log=log)
+    sc = mdas.Scene(3, 2, 5, float(W), float(H))
+    passes = g.Run(sc, target)
+    g.integrate_image(W, H)
+    assert _log_values(log, "INITIAL SAMPLES") == [4 << 9]
+    assert len(_log_values(log, "INITIAL SAMPLING TIME")) == 1
+    for key in ("COMPUTE ERRORS TIME", "SAMPLES", "ADAPTIVE SAMPLING TIME"):
+        assert len(_log_values(log, key)) == passes, key
+    assert len(_log_values(log, "TRACE TIME")) == passes + 1
+    assert sum(_log_values(log, "SAMPLES")) == g.GetNumberOfSamples() - (4 << 9)
+    assert _log_values(log, "TOTAL SAMPLES") == [g.GetNumberOfSamples()]
+    times = sum((_log_values(log, k) for k in ("COMPUTE ERRORS TIME", "ADAPTIVE SAMPLING TIME",
+                                                "TRACE TIME", "INTEGRATE TIME")), [])
+    assert all(t >= 0 for t in times) and sum(times) > 0

@@ -22,8 +22,10 @@ def test_rel_mse_closed_form_and_nan():
     img[0, 0, 0] = 0.4
     want = (0.1 ** 2 / (0.09 + 1e-2)) / 6
     assert np.isclose(metrics.rel_mse(img, ref), want)
-    img[0, 1, 1] = np.nan
-    assert np.isclose(metrics.rel_mse(img, ref), want)  # NaN -> 0
+    img[0, 1, 1] = np.nan  # a NaN pixel counts as 0: error ref^2
+    assert np.isclose(metrics.rel_mse(img, ref), want + (0.09 / (0.09 + 1e-2)) / 6)
+    assert np.isclose(metrics.mse(img, ref), (0.01 + 0.09) / 6)
+    assert not np.isnan(img[0, 0, 0]) and np.isnan(img[0, 1, 1])  # input untouched
 
 
 def test_pfm_and_ppm(tmp_path):
