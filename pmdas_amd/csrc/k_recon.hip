@@ -1,9 +1,9 @@
 // Reconstruction: box-filter of the leaves into pixels (kdgrid.cu:483-582),
 // done as a deterministic gather instead of the reference's float atomics.
 //
-//  1. k_recon_prep: one coalesced pass over the leaves writes a 48-byte
-//     record per leaf (xy box, mean value, volume, xy area, pixel span) and
-//     counts the pixel tiles each leaf's span touches.
+//  1. k_recon_prep: one coalesced pass over the leaves writes a 32-byte
+//     record per leaf (xy box, mean value, volume) and counts the pixel tiles
+//     each leaf's span touches.
 //  2. k_scan_keys / k_recon_emit: every leaf emits one 64-bit key per tile,
 //         key = morton2(tile_x, tile_y) << 32 | leaf index,
 //     in leaf order (a workgroup-chunked exclusive scan gives the offsets).
@@ -24,6 +24,7 @@
 namespace mdas {
 
 static const int kSortBlocks = 1024;
+static const int kSortMaxBits = 10;  // digit width limit (histograms of 1024 bins)
 
 struct ReconGeo {
     int W, H;
@@ -47,22 +48,24 @@ __device__ __forceinline__ void pixel_span(float mn0, float mx0, float mn1, floa
     y1 = min(yy + 1, G.H);
 }
 
+// The span with the start clamped to the image (a span starting past the
+// image is empty), from a record's xy box.
+__device__ __forceinline__ void record_span(const float4& q0, const ReconGeo& G, int& x0, int& x1,
+                                            int& y0, int& y1) {
+    pixel_span(q0.x, q0.y, q0.z, q0.w, G, x0, x1, y0, y1);
+    x0 = min(x0, G.W);
+    y0 = min(y0, G.H);
+}
+
 __device__ __forceinline__ int tiles_of(int x0, int x1, int y0, int y1, const ReconGeo& G) {
     if (x0 >= x1 || y0 >= y1) return 0;
     return ((x1 - 1) / G.TW - x0 / G.TW + 1) * ((y1 - 1) / G.TH - y0 / G.TH + 1);
 }
 
-__device__ __forceinline__ void unpack_span(const float4& q2, int& x0, int& x1, int& y0, int& y1) {
-    const unsigned sx = __float_as_uint(q2.y), sy = __float_as_uint(q2.z);
-    x0 = (int)(sx & 0xffffu);
-    x1 = (int)(sx >> 16);
-    y0 = (int)(sy & 0xffffu);
-    y1 = (int)(sy >> 16);
-}
-
-// Record of leaf i: q0 = (mn0, mx0, mn1, mx1), q1 = (mean value, volume),
-// q2 = ((mx0-mn0)*(mx1-mn1), x0 | x1 << 16, y0 | y1 << 16, 0).  The mean is
-// the slot-order sum times 1/n (kdgrid.cu:510-518, vec_math.h:492-496).
+// Record of leaf i: q0 = (mn0, mx0, mn1, mx1), q1 = (mean value, volume).
+// The mean is the slot-order sum times 1/n (kdgrid.cu:510-518,
+// vec_math.h:492-496); the pixel span and the xy area are recomputed from q0
+// by whoever needs them, with the same operations.
 template <int D>
 __global__ __launch_bounds__(256) void k_recon_prep(DevGrid a, ReconGeo G) {
     __shared__ int red[4];
@@ -90,18 +93,13 @@ __global__ __launch_bounds__(256) void k_recon_prep(DevGrid a, ReconGeo G) {
         float vol = 1.0f;
 #pragma unroll
         for (int d = 0; d < D; ++d) vol *= (mx[d] - mn[d]);
+        const float4 q0 = make_float4(mn[0], mx[0], mn[1], mx[1]);
         int x0, x1, y0, y1;
-        pixel_span(mn[0], mx[0], mn[1], mx[1], G, x0, x1, y0, y1);
-        // spans that start past the image are empty; clamp so they pack
-        x0 = min(x0, G.W);
-        y0 = min(y0, G.H);
+        record_span(q0, G, x0, x1, y0, y1);
         c += tiles_of(x0, x1, y0, y1, G);
-        float4* r = a.rrec + (size_t)i * 3;
-        r[0] = make_float4(mn[0], mx[0], mn[1], mx[1]);
+        float4* r = a.rrec + (size_t)i * 2;
+        r[0] = q0;
         r[1] = make_float4(sx * inv, sy * inv, sz * inv, vol);
-        r[2] = make_float4((mx[0] - mn[0]) * (mx[1] - mn[1]),
-                           __uint_as_float((unsigned)x0 | ((unsigned)x1 << 16)),
-                           __uint_as_float((unsigned)y0 | ((unsigned)y1 << 16)), 0.0f);
     }
     c = wave_sum(c);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
@@ -153,7 +151,7 @@ __global__ __launch_bounds__(256) void k_recon_emit(DevGrid a, ReconGeo G,
         const int i = t0 + threadIdx.x;
         int x0 = 0, x1 = 0, y0 = 0, y1 = 0, c = 0;
         if (i < end) {
-            unpack_span(a.rrec[(size_t)i * 3 + 2], x0, x1, y0, y1);
+            record_span(a.rrec[(size_t)i * 2], G, x0, x1, y0, y1);
             c = tiles_of(x0, x1, y0, y1, G);
         }
         int total;
@@ -215,49 +213,57 @@ __global__ __launch_bounds__(1024) void k_scan_down(unsigned* data, int n, const
 }
 
 // ------------------------------------------------------------ radix sort --
-// Stable LSD radix sort of 64-bit keys over bits [shift, shift+8) per pass.
-// Reduce-then-scan: per-workgroup digit histograms, one exclusive scan over
-// (digit, workgroup), then a scatter in which every workgroup ranks a tile of
-// 256 keys by digit with wave ballots, stages the tile in LDS in digit order
-// and writes each digit run contiguously.
+// Stable LSD radix sort of 64-bit keys over bits [shift, shift+BITS) per
+// pass, BITS chosen per sort so the Morton bits take as few passes as
+// possible (at most 10 bits a pass).  Reduce-then-scan: per-workgroup digit
+// histograms, one exclusive scan over (digit, workgroup), then a scatter in
+// which every workgroup ranks a tile of 256 keys by digit with wave ballots,
+// stages the tile in LDS in digit order and writes each digit run
+// contiguously.
+template <int BITS>
 __global__ __launch_bounds__(256) void k_rs_hist(const unsigned long long* keys, const int* n_ptr,
                                                  int shift, unsigned* hist) {
-    __shared__ unsigned h[256];
+    constexpr int NB = 1 << BITS;
+    __shared__ unsigned h[NB];
     const int n = *n_ptr;
-    h[threadIdx.x] = 0;
+    for (int d = threadIdx.x; d < NB; d += 256) h[d] = 0;
     __syncthreads();
     int beg, end;
     chunk_range(n, gridDim.x, blockIdx.x, beg, end);
     for (int i = beg + threadIdx.x; i < end; i += 256)
-        atomicAdd(&h[(unsigned)(keys[i] >> shift) & 255u], 1u);
+        atomicAdd(&h[(unsigned)(keys[i] >> shift) & (unsigned)(NB - 1)], 1u);
     __syncthreads();
-    hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+    for (int d = threadIdx.x; d < NB; d += 256) hist[d * gridDim.x + blockIdx.x] = h[d];
 }
 
+template <int BITS>
 __global__ __launch_bounds__(256) void k_rs_scatter(const unsigned long long* in,
                                                     unsigned long long* out, const int* n_ptr,
                                                     int shift, const unsigned* hist) {
-    __shared__ unsigned base[256];
-    __shared__ int wcnt[4][256];
-    __shared__ int toff[256];
-    __shared__ int ttot[256];
+    constexpr int NB = 1 << BITS;
+    constexpr int R = NB > 256 ? NB / 256 : 1;  // digits per thread in the tile scan
+    __shared__ unsigned base[NB];
+    __shared__ int wcnt[4][NB];
+    __shared__ int toff[NB];
+    __shared__ int ttot[NB];
     __shared__ int scr[4];
     __shared__ unsigned long long stage[256];
     const int n = *n_ptr;
     const int w = threadIdx.x >> 6;
-    base[threadIdx.x] = hist[threadIdx.x * gridDim.x + blockIdx.x];
+    for (int d = threadIdx.x; d < NB; d += 256) base[d] = hist[d * gridDim.x + blockIdx.x];
     int beg, end;
     chunk_range(n, gridDim.x, blockIdx.x, beg, end);
     for (int t0 = beg; t0 < end; t0 += 256) {
         const int i = t0 + threadIdx.x;
         const bool valid = i < end;
         const unsigned long long k = valid ? in[i] : 0ull;
-        const unsigned d = (unsigned)(k >> shift) & 255u;
-        for (int q = 0; q < 4; ++q) wcnt[q][threadIdx.x] = 0;
+        const unsigned d = (unsigned)(k >> shift) & (unsigned)(NB - 1);
+        for (int q = 0; q < 4; ++q)
+            for (int dd = threadIdx.x; dd < NB; dd += 256) wcnt[q][dd] = 0;
         __syncthreads();
         unsigned long long m = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < BITS; ++b) {
             const bool bit = (d >> b) & 1u;
             const unsigned long long bal = __ballot(bit);
             m &= bit ? bal : ~bal;
@@ -266,11 +272,24 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const unsigned long long* in
         if (valid && rw == 0) wcnt[w][d] = __popcll(m);
         __syncthreads();
         {
-            const int t = threadIdx.x;
-            int tot = wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+            int loc[R], sum = 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int dd = (int)threadIdx.x * R + r;
+                loc[r] = dd < NB ? wcnt[0][dd] + wcnt[1][dd] + wcnt[2][dd] + wcnt[3][dd] : 0;
+                sum += loc[r];
+            }
             int all;
-            toff[t] = block_exscan(tot, scr, all);
-            ttot[t] = tot;
+            int ex = block_exscan(sum, scr, all);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int dd = (int)threadIdx.x * R + r;
+                if (dd < NB) {
+                    toff[dd] = ex;
+                    ttot[dd] = loc[r];
+                    ex += loc[r];
+                }
+            }
         }
         __syncthreads();
         const int nvalid = min(256, end - t0);
@@ -282,11 +301,11 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const unsigned long long* in
         __syncthreads();
         if ((int)threadIdx.x < nvalid) {
             const unsigned long long kk = stage[threadIdx.x];
-            const unsigned dd = (unsigned)(kk >> shift) & 255u;
+            const unsigned dd = (unsigned)(kk >> shift) & (unsigned)(NB - 1);
             out[base[dd] + (threadIdx.x - toff[dd])] = kk;
         }
         __syncthreads();
-        base[threadIdx.x] += ttot[threadIdx.x];
+        for (int dd = threadIdx.x; dd < NB; dd += 256) base[dd] += ttot[dd];
         __syncthreads();
     }
 }
@@ -298,24 +317,45 @@ static void scan_u32(unsigned* data, int n, unsigned* sums, hipStream_t st) {
     hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(1024), 0, st, data, n, sums);
 }
 
-// Sorts keys[0, n) by bits [lo, hi); returns the buffer holding the result.
+template <int BITS>
+static void sort_pass(Grid& g, const unsigned long long* a, unsigned long long* b, int sh,
+                      hipStream_t st) {
+    const int* n_ptr = &g.dev.cnt->n_keys;
+    const int nh = (1 << BITS) * kSortBlocks;
+    kbeg(g, st, "k_rs_hist");
+    hipLaunchKernelGGL(k_rs_hist<BITS>, dim3(kSortBlocks), dim3(256), 0, st, a, n_ptr, sh, g.rs_hist);
+    kend(g, st);
+    kbeg(g, st, "k_rs_scan");
+    scan_u32(g.rs_hist, nh, g.rs_hist + nh, st);
+    kend(g, st);
+    kbeg(g, st, "k_rs_scatter");
+    hipLaunchKernelGGL(k_rs_scatter<BITS>, dim3(kSortBlocks), dim3(256), 0, st, a, b, n_ptr, sh,
+                       g.rs_hist);
+    kend(g, st);
+}
+
+// Sorts keys[0, n) by bits [lo, hi) in ceil((hi - lo) / 10) passes of equal
+// digit width; returns the buffer holding the result.
 static unsigned long long* radix_sort_keys(Grid& g, int lo, int hi, hipStream_t st) {
     unsigned long long* a = g.keys;
     unsigned long long* b = g.keys_alt;
-    const int* n_ptr = &g.dev.cnt->n_keys;
     g.recon_sort_passes = 0;
-    for (int sh = lo; sh < hi; sh += 8) {
-        kbeg(g, st, "k_rs_hist");
-        hipLaunchKernelGGL(k_rs_hist, dim3(kSortBlocks), dim3(256), 0, st, a, n_ptr, sh, g.rs_hist);
-        kend(g, st);
-        kbeg(g, st, "k_rs_scan");
-        scan_u32(g.rs_hist, 256 * kSortBlocks, g.rs_hist + 256 * kSortBlocks, st);
-        kend(g, st);
-        kbeg(g, st, "k_rs_scatter");
-        hipLaunchKernelGGL(k_rs_scatter, dim3(kSortBlocks), dim3(256), 0, st, a, b, n_ptr, sh,
-                           g.rs_hist);
-        kend(g, st);
+    const int total = hi - lo;
+    if (total <= 0) return a;
+    const int passes = (total + kSortMaxBits - 1) / kSortMaxBits;
+    const int bits = max(4, (total + passes - 1) / passes);
+    for (int sh = lo; sh < hi; sh += bits) {
+        switch (bits) {
+        case 4: sort_pass<4>(g, a, b, sh, st); break;
+        case 5: sort_pass<5>(g, a, b, sh, st); break;
+        case 6: sort_pass<6>(g, a, b, sh, st); break;
+        case 7: sort_pass<7>(g, a, b, sh, st); break;
+        case 8: sort_pass<8>(g, a, b, sh, st); break;
+        case 9: sort_pass<9>(g, a, b, sh, st); break;
+        default: sort_pass<10>(g, a, b, sh, st); break;
+        }
         g.recon_sort_passes++;
+        g.recon_sort_bits = bits;
         unsigned long long* t = a;
         a = b;
         b = t;
@@ -366,18 +406,24 @@ __global__ __launch_bounds__(256) void k_gather(DevGrid a, ReconGeo G,
         const int nrec = max(0, min(GS, s1 - base));
         if (gl < nrec) {
             const int i = (int)(unsigned)(keys[base + gl] & 0xffffffffull);
-            const float4* r = a.rrec + (size_t)i * 3;
-            r0[w][gbase + gl] = r[0];
+            const float4* r = a.rrec + (size_t)i * 2;
+            const float4 q0 = r[0];
+            int x0, x1, y0, y1;
+            record_span(q0, G, x0, x1, y0, y1);
+            r0[w][gbase + gl] = q0;
             r1[w][gbase + gl] = r[1];
-            r2[w][gbase + gl] = r[2];
+            r2[w][gbase + gl] = make_float4((q0.y - q0.x) * (q0.w - q0.z),
+                                            __uint_as_float((unsigned)x0 | ((unsigned)x1 << 16)),
+                                            __uint_as_float((unsigned)y0 | ((unsigned)y1 << 16)), 0.0f);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         for (int j = 0; j < nrec; ++j) {
-            int x0, x1, y0, y1;
             const float4 q2 = r2[w][gbase + j];
-            unpack_span(q2, x0, x1, y0, y1);
+            const unsigned spx = __float_as_uint(q2.y), spy = __float_as_uint(q2.z);
+            const int x0 = (int)(spx & 0xffffu), x1 = (int)(spx >> 16);
+            const int y0 = (int)(spy & 0xffffu), y1 = (int)(spy >> 16);
             if (px >= x0 && px < x1 && py >= y0 && py < y1) {
                 const float4 q0 = r0[w][gbase + j];
                 const float4 q1 = r1[w][gbase + j];
@@ -542,7 +588,7 @@ hipError_t recon_prepare(Grid& g, int W, int H, hipStream_t st) {
             if (g.dev.rrec) (void)hipFree(g.dev.rrec);
             g.dev.rrec = nullptr;
             long long cap = std::min<long long>((long long)g.cap_l, (long long)L + L / 4 + 1024);
-            e0 = hipMalloc(&g.dev.rrec, sizeof(float4) * 3 * (size_t)cap);
+            e0 = hipMalloc(&g.dev.rrec, sizeof(float4) * 2 * (size_t)cap);
             if (e0 != hipSuccess) {
                 g.rrec_cap = 0;
                 return e0;
@@ -586,7 +632,7 @@ hipError_t recon_prepare(Grid& g, int W, int H, hipStream_t st) {
     }
     if (!g.rs_hist) {
         // histograms plus the block sums of their scan
-        e = hipMalloc(&g.rs_hist, sizeof(unsigned) * (256 * kSortBlocks + 1024));
+        e = hipMalloc(&g.rs_hist, sizeof(unsigned) * ((1 << kSortMaxBits) * kSortBlocks + 1024));
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

@@ -758,7 +758,7 @@ int mdas_get_pass_stats(mdas_handle h, int* out, int max_rows) {
 // coordinates (4*D per sample) and values (12), leaf records of LB = 48 or 64
 // bytes (slots + box, read or written whole), the per-leaf copies of the
 // leaf's points (4*D per point), 4-byte errors and LCG states, 1-byte flags,
-// 8-byte keys, 48-byte reconstruction records, float4 pixels.
+// 8-byte keys, 32-byte reconstruction records, float4 pixels.
 static double kernel_bytes(const Grid& g, const std::string& k) {
     const double D = g.D;
     const double LB = 16.0 * mdas::leaf_quads(g.D);
@@ -787,14 +787,14 @@ static double kernel_bytes(const Grid& g, const std::string& k) {
     }
     const double L = (double)g.host.n_leaves, N = (double)g.host.n_samples;
     const double K = (double)g.last_n_keys;
-    if (k == "k_recon_prep") return L * (LB + 48.0) + 12.0 * N;
+    if (k == "k_recon_prep") return L * (LB + 32.0) + 12.0 * N;
     if (k == "k_scan_keys") return (16.0 + 8.0) * g.chunk_blocks;
     if (k == "k_recon_emit") return L * 16.0 + K * 8.0;
     if (k == "k_rs_hist") return K * 8.0 * g.recon_sort_passes;
-    if (k == "k_rs_scan") return 3.0 * 4.0 * 256 * 1024 * g.recon_sort_passes;
+    if (k == "k_rs_scan") return 3.0 * 4.0 * (1 << g.recon_sort_bits) * 1024 * g.recon_sort_passes;
     if (k == "k_rs_scatter") return K * 16.0 * g.recon_sort_passes;
     if (k == "k_tile_ranges") return K * 8.0;
-    if (k == "k_gather") return K * (8.0 + 48.0) + 16.0 * (double)g.recon_pixels;
+    if (k == "k_gather") return K * (8.0 + 32.0) + 16.0 * (double)g.recon_pixels;
     return b;
 }
 

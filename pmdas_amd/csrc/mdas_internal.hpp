@@ -68,7 +68,7 @@ struct DevGrid {
     int4* part;            // [kChunkBlocks] per-workgroup (selected, split, samples, carry)
     int2* part_off;        // [kChunkBlocks] exclusive prefix of (selected, split)
     uint4* epart;          // [kErrBlocks] per-workgroup (max error bits, stale, values)
-    float4* rrec;          // [cap_l * 3]  reconstruction record per leaf
+    float4* rrec;          // [L * 2]      reconstruction record per leaf
     unsigned char* rcnt;   // [cap_l]      reconstruction tiles per leaf
     DevCounters* cnt;
     PassStat* pstats;
@@ -126,7 +126,7 @@ struct Grid {
     int* tile_start = nullptr;
     int* tile_end = nullptr;
     long long tiles_cap = 0;
-    unsigned* rs_hist = nullptr;  // radix histogram [256 * kSortBlocks]
+    unsigned* rs_hist = nullptr;  // radix histogram [1024 * kSortBlocks] + scan sums
     int tile_w = 8, tile_h = 8;
     long long last_n_keys = 0;
     long long rrec_cap = 0;       // leaves the reconstruction records can hold
@@ -148,6 +148,7 @@ struct Grid {
     std::vector<PassStat> pstats_host;
     long long recon_pixels = 0;
     int recon_sort_passes = 0;
+    int recon_sort_bits = 8;  // digit width of the last reconstruction sort
     // launch shape of the per-pass kernels (mdas_set_launch_shape)
     int chunk_blocks = 4096;   // leaf chunks of selection / compaction / reconstruction
     int err_blocks = 1024;     // workgroups of the error pass
