@@ -372,7 +372,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the sampler is single-GPU; "
-              "sharding by image / screen tile is described in DESIGN.md)", file=sys.stderr)
+              "sharding by image or screen tile is a design, not built)", file=sys.stderr)
         sys.exit(2)
     log = lambda m: print(m, file=sys.stderr, flush=True)
     import pmdas_amd

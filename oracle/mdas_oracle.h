@@ -4,7 +4,8 @@
  * A deterministic OpenMP C++ restatement of the reference's KDGrid
  * (SDK/lib/mdas/kdgrid.cu, kdgrid.h, aabb.h, point.h, cuda/random.h,
  * cuda/helpers.h).  Only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg may load it, and only as the checker / CPU baseline.
+ * cpu_baseline leg may load it, and only to check results and as the CPU
+ * baseline.
  *
  * Semantics fixed here (the reference is nondeterministic and fast-math):
  *   - IEEE-754 FP32, correctly rounded + - * / sqrt, no FP contraction;

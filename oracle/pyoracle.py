@@ -1,7 +1,8 @@
 """ctypes wrapper of the CPU oracle (liboracle_mdas.so).
 
 TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg use it, as the checker / CPU baseline -- never the product.
+cpu_baseline leg use it, to check results and as the CPU baseline -- never
+the product.
 """
 import ctypes
 import os
