@@ -278,24 +278,42 @@ __device__ __forceinline__ void select_body(const DevGrid& a, int blk, int nblk)
     chunk_range(L, nblk, blk, beg, end);
     int ns = 0, nl = 0, nc = 0;
     float carry = 0.0f;
-    for (int i = beg + threadIdx.x; i < end; i += 256) {
-        const float e = a.err[i];
-        unsigned s = a.seed[i];
-        if (s == 0u) s = tea4((unsigned)i, 0u);
-        const float u = rnd(s);
-        const bool sel = select_test(e / gmax, alpha, u);
-        const unsigned cnt = a.flags[i] & 7u;
-        const bool split = sel && cnt == 4u;
-        a.flags[i] = (unsigned char)(cnt | (sel ? 8u : 0u) | (split ? 16u : 0u));
-        // a selected leaf's state after the placement's candidatesNum * D
-        // draws, by the LCG's affine jump-ahead; the placement recovers the
-        // state it starts from with the inverse map
-        if (sel) s = a.jmul * s + a.jadd;
-        a.seed[i] = s;
-        ns += sel ? 1 : 0;
-        nl += split ? 1 : 0;
-        nc += sel ? (int)cnt : 0;
-        if (!split) carry = fmaxf(carry, e);
+    // four leaves per thread and step, their loads issued together
+    constexpr int U = 4;
+    for (int i0 = beg + threadIdx.x; i0 < end; i0 += U * 256) {
+        float ev[U];
+        unsigned sv[U], fv[U];
+#pragma unroll
+        for (int r = 0; r < U; ++r) {
+            const int i = i0 + r * 256;
+            if (i < end) {
+                ev[r] = a.err[i];
+                sv[r] = a.seed[i];
+                fv[r] = a.flags[i];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < U; ++r) {
+            const int i = i0 + r * 256;
+            if (i >= end) continue;
+            const float e = ev[r];
+            unsigned s = sv[r];
+            if (s == 0u) s = tea4((unsigned)i, 0u);
+            const float u = rnd(s);
+            const bool sel = select_test(e / gmax, alpha, u);
+            const unsigned cnt = fv[r] & 7u;
+            const bool split = sel && cnt == 4u;
+            a.flags[i] = (unsigned char)(cnt | (sel ? 8u : 0u) | (split ? 16u : 0u));
+            // a selected leaf's state after the placement's candidatesNum * D
+            // draws, by the LCG's affine jump-ahead; the placement recovers
+            // the state it starts from with the inverse map
+            if (sel) s = a.jmul * s + a.jadd;
+            a.seed[i] = s;
+            ns += sel ? 1 : 0;
+            nl += split ? 1 : 0;
+            nc += sel ? (int)cnt : 0;
+            if (!split) carry = fmaxf(carry, e);
+        }
     }
     ns = wave_sum(ns);
     nl = wave_sum(nl);
@@ -583,18 +601,31 @@ __device__ __forceinline__ void compact_body(const DevGrid& a, int blk, int nblk
     int beg, end;
     chunk_range(L, nblk, blk, beg, end);
     int2 base = a.part_off[blk];
-    for (int t0 = beg; t0 < end; t0 += 256) {
-        const int i = t0 + threadIdx.x;
-        const unsigned f = i < end ? a.flags[i] : 0u;
-        const bool sel = (f & 8u) != 0u, split = (f & 16u) != 0u;
-        const unsigned cnt = f & 7u;
-        int rs, rl, ts, tl;
-        block_rank2(sel, split, lds, rs, rl, ts, tl);
-        if (sel)
-            a.qlist[base.x + rs] = make_int4(i, N0 + base.x + rs, split ? L + base.y + rl : -1, (int)cnt);
-        if (i < end) a.flags[i] = (unsigned char)(split ? 2u : (sel ? cnt + 1u : cnt));
-        base.x += ts;
-        base.y += tl;
+    constexpr int U = 4;  // tiles whose flags are loaded together
+    for (int t00 = beg; t00 < end; t00 += U * 256) {
+        unsigned fv[U];
+#pragma unroll
+        for (int r = 0; r < U; ++r) {
+            const int i = t00 + r * 256 + threadIdx.x;
+            fv[r] = i < end ? a.flags[i] : 0u;
+        }
+#pragma unroll
+        for (int r = 0; r < U; ++r) {
+            const int t0 = t00 + r * 256;
+            if (t0 >= end) break;  // uniform across the workgroup
+            const int i = t0 + threadIdx.x;
+            const unsigned f = fv[r];
+            const bool sel = (f & 8u) != 0u, split = (f & 16u) != 0u;
+            const unsigned cnt = f & 7u;
+            int rs, rl, ts, tl;
+            block_rank2(sel, split, lds, rs, rl, ts, tl);
+            if (sel)
+                a.qlist[base.x + rs] =
+                    make_int4(i, N0 + base.x + rs, split ? L + base.y + rl : -1, (int)cnt);
+            if (i < end) a.flags[i] = (unsigned char)(split ? 2u : (sel ? cnt + 1u : cnt));
+            base.x += ts;
+            base.y += tl;
+        }
     }
 }
 
