@@ -374,6 +374,12 @@ __global__ __launch_bounds__(256) void k_tile_ranges(const unsigned long long* k
 }
 
 // ---------------------------------------------------------------- gather --
+__device__ __forceinline__ bool inside_span(const float4& q2, int px, int py) {
+    const unsigned spx = __float_as_uint(q2.y), spy = __float_as_uint(q2.z);
+    return px >= (int)(spx & 0xffffu) && px < (int)(spx >> 16) && py >= (int)(spy & 0xffffu) &&
+           py < (int)(spy >> 16);
+}
+
 // GS = pixels per tile (16 or 64); a wave holds 64/GS tiles.
 template <int GS>
 __global__ __launch_bounds__(256) void k_gather(DevGrid a, ReconGeo G,
@@ -381,6 +387,7 @@ __global__ __launch_bounds__(256) void k_gather(DevGrid a, ReconGeo G,
                                                 const int* __restrict__ tstart,
                                                 const int* __restrict__ tend, float4* pixels) {
     __shared__ float4 r0[4][64], r1[4][64], r2[4][64];
+    __shared__ float2 r3[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int grp = lane / GS, gl = lane % GS;      // tile within the wave, lane within the tile
     const int tile = (blockIdx.x * 4 + w) * (64 / GS) + grp;
@@ -395,46 +402,70 @@ __global__ __launch_bounds__(256) void k_gather(DevGrid a, ReconGeo G,
         s0 = tstart[code];
         s1 = tend[code];
     }
-    // rounds of GS records until the longest tile of the wave is done
+    // rounds of GS records until the longest tile of the wave is done; the
+    // records of round rd + 1 and the keys of round rd + 2 are loaded while
+    // round rd is accumulated
     int rounds = (s1 - s0 + GS - 1) / GS;
 #pragma unroll
     for (int o = GS; o < 64; o <<= 1) rounds = max(rounds, __shfl_xor(rounds, o, 64));
     float accx = 0.0f, accy = 0.0f, accz = 0.0f;
     const int gbase = grp * GS;
+    auto nrec_of = [&](int rd) { return max(0, min(GS, s1 - (s0 + rd * GS))); };
+    float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = c0;  // records of the round to stage
+    if (gl < nrec_of(0)) {
+        const int i = (int)(unsigned)(keys[s0 + gl] & 0xffffffffull);
+        c0 = a.rrec[(size_t)i * 2];
+        c1 = a.rrec[(size_t)i * 2 + 1];
+    }
+    unsigned long long knext = gl < nrec_of(1) ? keys[s0 + GS + gl] : 0ull;
     for (int rd = 0; rd < rounds; ++rd) {
-        const int base = s0 + rd * GS;
-        const int nrec = max(0, min(GS, s1 - base));
+        const int nrec = nrec_of(rd);
         if (gl < nrec) {
-            const int i = (int)(unsigned)(keys[base + gl] & 0xffffffffull);
-            const float4* r = a.rrec + (size_t)i * 2;
-            const float4 q0 = r[0];
+            // per record, once: span, xy area, the span's first pixel bounds
+            // (kdgrid.cu:532-536, the same operations every lane would do)
+            // and the mask of the tile's pixels inside the span
             int x0, x1, y0, y1;
-            record_span(q0, G, x0, x1, y0, y1);
-            r0[w][gbase + gl] = q0;
-            r1[w][gbase + gl] = r[1];
-            r2[w][gbase + gl] = make_float4((q0.y - q0.x) * (q0.w - q0.z),
+            record_span(c0, G, x0, x1, y0, y1);
+            unsigned mask = 0u;  // tiles of up to 32 pixels; larger tiles test the span
+            if (GS <= 32) {
+                for (int k = 0; k < GS; ++k) {
+                    const int qx = tx * G.TW + k % G.TW, qy = ty * G.TH + k / G.TW;
+                    if (qx >= x0 && qx < x1 && qy >= y0 && qy < y1) mask |= 1u << k;
+                }
+            }
+            r0[w][gbase + gl] = c0;
+            r1[w][gbase + gl] = c1;
+            r2[w][gbase + gl] = make_float4((c0.y - c0.x) * (c0.w - c0.z),
                                             __uint_as_float((unsigned)x0 | ((unsigned)x1 << 16)),
-                                            __uint_as_float((unsigned)y0 | ((unsigned)y1 << 16)), 0.0f);
+                                            __uint_as_float((unsigned)y0 | ((unsigned)y1 << 16)),
+                                            __uint_as_float(mask));
+            r3[w][gbase + gl] = make_float2((float)x0 * G.sX / G.fW, (float)y0 * G.sY / G.fH);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (gl < nrec_of(rd + 1)) {
+            const int i = (int)(unsigned)(knext & 0xffffffffull);
+            c0 = a.rrec[(size_t)i * 2];
+            c1 = a.rrec[(size_t)i * 2 + 1];
+        }
+        knext = gl < nrec_of(rd + 2) ? keys[s0 + (rd + 2) * GS + gl] : 0ull;
         for (int j = 0; j < nrec; ++j) {
             const float4 q2 = r2[w][gbase + j];
-            const unsigned spx = __float_as_uint(q2.y), spy = __float_as_uint(q2.z);
-            const int x0 = (int)(spx & 0xffffu), x1 = (int)(spx >> 16);
-            const int y0 = (int)(spy & 0xffffu), y1 = (int)(spy >> 16);
-            if (px >= x0 && px < x1 && py >= y0 && py < y1) {
+            if (GS <= 32 ? ((__float_as_uint(q2.w) >> gl) & 1u) != 0u : inside_span(q2, px, py)) {
+                const unsigned spx = __float_as_uint(q2.y), spy = __float_as_uint(q2.z);
+                const int x0 = (int)(spx & 0xffffu), y0 = (int)(spy & 0xffffu);
                 const float4 q0 = r0[w][gbase + j];
                 const float4 q1 = r1[w][gbase + j];
+                const float2 q3 = r3[w][gbase + j];
                 // accumulated pixel bounds exactly as kdgrid.cu:536-576 walks them
-                float pMinX = (float)x0 * G.sX / G.fW;
+                float pMinX = q3.x;
                 float pMaxX = pMinX + G.stepX;
                 for (int q = x0; q < px; ++q) {
                     pMinX = pMaxX;
                     pMaxX += G.stepX;
                 }
-                float pMinY = (float)y0 * G.sY / G.fH;
+                float pMinY = q3.y;
                 float pMaxY = pMinY + G.stepY;
                 for (int q = y0; q < py; ++q) {
                     pMinY = pMaxY;
