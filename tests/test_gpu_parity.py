@@ -169,10 +169,15 @@ def test_integrate_other_resolution(mdas):
     g, o, sc, target = _make(mdas, 2, 4, 1, 2, 40, 30, 8, 0.25, 4)
     g.Run(sc, target)
     o.run(sc.params_bytes(), target)
-    for W, H in [(80, 60), (20, 15), (37, 23)]:
+    for W, H in [(80, 60), (20, 15), (37, 23), (1, 1), (7, 3), (1, 45), (300, 2)]:
         _say(f"integrate {W}x{H}")
         px = g.integrate_image(W, H)
         assert np.array_equal(px.view(np.uint32), o.integrate(W, H).view(np.uint32))
+        for tile in ((8, 8), (4, 4), (16, 1)):
+            g.set_recon_tile(*tile)
+            assert np.array_equal(g.integrate_image(W, H).view(np.uint32),
+                                  px.view(np.uint32)), (W, H, tile)
+        g.set_recon_tile(8, 8)
 
 
 def test_autotune_keeps_result(mdas):
