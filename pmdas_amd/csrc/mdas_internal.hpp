@@ -127,7 +127,7 @@ struct Grid {
     int* tile_end = nullptr;
     long long tiles_cap = 0;
     unsigned* rs_hist = nullptr;  // radix histogram [1024 * kSortBlocks] + scan sums
-    int tile_w = 8, tile_h = 8;
+    int tile_w = 4, tile_h = 4;  // reconstruction tile (mdas_set_recon_tile / autotune)
     long long last_n_keys = 0;
     long long rrec_cap = 0;       // leaves the reconstruction records can hold
     int* d_scalar = nullptr;      // small device scratch
