@@ -590,7 +590,8 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
 // Compaction: the selected leaves of a workgroup's chunk (the selection's
 // partition) are ranked in leaf order by wave ballots; entry r of the
 // placement list (r = the leaf's rank among all selected leaves, so its new
-// sample is N_old + r) gets (leaf, sample, right child or -1, count).  Every
+// sample is N_old + r) gets (leaf | count << 30, right child or -1); leaf
+// indices are below 2^30, and a split leaf's count is 4.  Every
 // leaf's count is rewritten here as it stands after the placement, in full
 // coalesced lines.
 __device__ __forceinline__ void compact_body(const DevGrid& a, int blk, int nblk) {
@@ -619,9 +620,9 @@ __device__ __forceinline__ void compact_body(const DevGrid& a, int blk, int nblk
             const unsigned cnt = f & 7u;
             int rs, rl, ts, tl;
             block_rank2(sel, split, lds, rs, rl, ts, tl);
-            if (sel)
+            if (sel)  // the new sample is N0 + list index; a split leaf holds 4
                 a.qlist[base.x + rs] =
-                    make_int4(i, N0 + base.x + rs, split ? L + base.y + rl : -1, (int)cnt);
+                    make_int2(i | (int)((cnt & 3u) << 30), split ? L + base.y + rl : -1);
             if (i < end) a.flags[i] = (unsigned char)(split ? 2u : (sel ? cnt + 1u : cnt));
             base.x += ts;
             base.y += tl;
@@ -639,10 +640,12 @@ template <int D>
 __device__ __forceinline__ void place_body(const DevGrid& a, int blk, int nblk) {
     if (a.cnt->done) return;
     const int L = a.cnt->n_leaves_old;
+    const int N0 = a.cnt->n_samples_old;
     const int n = a.cnt->new_samples;
     for (int j = blk * 256 + threadIdx.x; j < n; j += nblk * 256) {
-        const int4 q = a.qlist[j];
-        place_leaf<D>(a, q.x, q.y, q.z, (unsigned)q.w, L);
+        const int2 q = a.qlist[j];
+        const unsigned cnt = q.y >= 0 ? 4u : ((unsigned)q.x >> 30);
+        place_leaf<D>(a, q.x & 0x3fffffff, N0 + j, q.y, cnt, L);
     }
 }
 

@@ -778,7 +778,7 @@ static double kernel_bytes(const Grid& g, const std::string& k) {
             b += (16.0 + 8.0) * g.chunk_blocks;
         else if (k == "k_place")  // compaction + list placement
             b += 2.0 * p.L +                                              // flags read, count written
-                 (double)p.sel * (16.0 + 16.0 + 4.0 + LB + 4.0 * D) +     // list written + read, state,
+                 (double)p.sel * (8.0 + 8.0 + 4.0 + LB + 4.0 * D) +       // list written + read, state,
                                                                           // record, new sample
                  4.0 * D * p.selcnt +                                     // the leaf's points
                  (double)(p.sel - p.split) * (16.0 + 4.0 * D) +           // slots + point of an insert

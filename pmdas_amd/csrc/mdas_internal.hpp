@@ -60,7 +60,7 @@ struct DevGrid {
     float* values;         // [cap_s * 3]  float3 AoS
     float4* leaf;          // [cap_l * leaf_quads(D)]  slots (~D = empty) + box
     float* pts;            // [cap_l * 4 * D]  copy of each leaf's sample points by slot
-    int4* qlist;           // [cap_l]      placement list (leaf, sample, right child, count)
+    int2* qlist;           // [cap_l]      placement list (leaf | count << 30, right child)
     float* err;            // [cap_l]      < 0 = stale
     unsigned* seed;        // [cap_l]      LCG state, 0 = unset
     unsigned char* flags;  // [cap_l]      bits 0-2 count, 3 selected, 4 split
