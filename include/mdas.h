@@ -288,6 +288,10 @@ int mdas_set_learned(mdas_handle h, int mode, mdas_mlp_handle m, float* rows, fl
  * the network's score s becomes s^kappa, i.e. selection tests
  * pow(s / max s, kappa * alpha) >= u -- the learned form's own flattening. */
 int mdas_set_learned_exponent(mdas_handle h, float kappa);
+/* Separate exponents: score = e^kappa_e * 2^(kappa_y * prediction), each in
+ * [0, 16], not both 0 (kappa_y = 0: the network is ignored; the plain error
+ * with the effective exponent kappa_e * alpha).  Reset by mdas_set_learned. */
+int mdas_set_learned_exponents(mdas_handle h, float kappa_e, float kappa_y);
 /* Rows offered since mdas_set_learned (those beyond cap_rows were dropped). */
 int mdas_get_learned_rows(mdas_handle h, long long* offered);
 /* Features and plain errors of leaves [0, n) after a learned-mode run. */

@@ -157,6 +157,7 @@ SIGNATURES = {
     "mdas_mlp_train": (_I, [_H, _P, _P, _LL, _I, _I, _F, ctypes.c_uint32, _P, _P]),
     "mdas_set_learned": (_I, [_H, _I, _H, _P, _P, _LL]),
     "mdas_set_learned_exponent": (_I, [_H, _F]),
+    "mdas_set_learned_exponents": (_I, [_H, _F, _F]),
     "mdas_get_learned_rows": (_I, [_H, ctypes.POINTER(_LL)]),
     "mdas_get_leaf_features": (_I, [_H, _P, _P, _LL]),
     "mdas_last_error": (ctypes.c_char_p, []),

@@ -12,10 +12,10 @@
 // Score used for selection: e * 2^clamp(y), y the network's prediction of
 // log2 r; the plain error e is the fallback when the network is absent or its
 // output is not finite.  Second label form: log2 of the squared image change
-// the split makes (image_label below), scored as 2^clamp(y).  A score
-// exponent kappa (default 1) replaces the score by score^kappa, so the
-// selection test pow(score / max, alpha) uses the effective exponent
-// kappa * alpha.
+// the split makes (image_label below), scored as 2^clamp(y).  In general the
+// score is e^kappa_e * 2^(kappa_y y) -- (1, 1) for the first form, (0, 1)
+// for the second; equal exponents kappa scale the selection test
+// pow(score / max, alpha) to the effective exponent kappa * alpha.
 #include "mdas_leaf.hpp"
 #include "mlp_device.hpp"
 
@@ -194,10 +194,12 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
                         if (fabsf(yv) < 3.0e38f) {  // false for NaN and +-inf
                             const float yc = fminf(fmaxf(yv, l.ylo), l.yhi);
                             float s;
-                            if (l.kappa == 1.0f)
-                                s = l.label_mode == 0 ? e[h] * exp2f(yc) : exp2f(yc);
-                            else  // (score)^kappa
-                                s = exp2f(l.kappa * (l.label_mode == 0 ? log2_floor(e[h]) + yc : yc));
+                            if (l.kappa_e == 1.0f && l.kappa_y == 1.0f)
+                                s = e[h] * exp2f(yc);
+                            else if (l.kappa_e == 0.0f && l.kappa_y == 1.0f)
+                                s = exp2f(yc);
+                            else  // e^kappa_e * 2^(kappa_y y)
+                                s = exp2f(l.kappa_e * log2_floor(e[h]) + l.kappa_y * yc);
                             if (s < 3.0e38f) q[h] = s;
                         }
                     }

@@ -105,7 +105,7 @@ struct LearnDev {
     MlpDev mlp;          // mlp.p == null: the score is the plain error
     float ylo, yhi;      // clamp of the network output
     int label_mode;      // 0: relative error reduction, score e 2^y; 1: image change, score 2^y
-    float kappa;         // score exponent (1: the score itself)
+    float kappa_e, kappa_y;  // score = e^kappa_e * 2^(kappa_y * y)
 };
 
 struct Grid {

@@ -275,7 +275,8 @@ int mdas_set_learned(mdas_handle h, int mode, mdas_mlp_handle m, float* rows, fl
     g.learn.cap_rows = (mode & MDAS_LEARN_COLLECT) ? cap_rows : 0;
     g.learn.mlp = (mode & MDAS_LEARN_SCORE) ? m->dev() : mdas::MlpDev{};
     g.learn.label_mode = (mode & MDAS_LEARN_IMAGE_LABEL) ? 1 : 0;
-    g.learn.kappa = 1.0f;
+    g.learn.kappa_e = g.learn.label_mode ? 0.0f : 1.0f;
+    g.learn.kappa_y = 1.0f;
     g.learn.ylo = g.learn.label_mode ? -120.0f : -10.0f;
     g.learn.yhi = g.learn.label_mode ? 120.0f : 0.0f;
     if (g.learn.rows) HIP_OK(hipMemset(g.learn.rows, 0, sizeof(long long)));
@@ -285,7 +286,18 @@ int mdas_set_learned(mdas_handle h, int mode, mdas_mlp_handle m, float* rows, fl
 int mdas_set_learned_exponent(mdas_handle h, float kappa) {
     if (!h) return set_error(MDAS_ERR_INVALID, "null handle");
     if (!(kappa > 0.0f) || kappa > 16.0f) return set_error(MDAS_ERR_INVALID, "kappa must be in (0, 16]");
-    h->g.learn.kappa = kappa;
+    h->g.learn.kappa_e = h->g.learn.label_mode ? 0.0f : kappa;
+    h->g.learn.kappa_y = kappa;
+    return MDAS_OK;
+}
+
+int mdas_set_learned_exponents(mdas_handle h, float kappa_e, float kappa_y) {
+    if (!h) return set_error(MDAS_ERR_INVALID, "null handle");
+    if (!(kappa_e >= 0.0f) || kappa_e > 16.0f || !(kappa_y >= 0.0f) || kappa_y > 16.0f ||
+        kappa_e + kappa_y == 0.0f)
+        return set_error(MDAS_ERR_INVALID, "exponents must be in [0, 16], not both 0");
+    h->g.learn.kappa_e = kappa_e;
+    h->g.learn.kappa_y = kappa_y;
     return MDAS_OK;
 }
 

@@ -314,6 +314,9 @@ class KDGrid:
     def set_learned_exponent(self, kappa):
         check(_lib.load().mdas_set_learned_exponent(self._h, kappa))
 
+    def set_learned_exponents(self, kappa_e, kappa_y):
+        check(_lib.load().mdas_set_learned_exponents(self._h, kappa_e, kappa_y))
+
     def learned_rows(self):
         n = ctypes.c_longlong(0)
         check(_lib.load().mdas_get_learned_rows(self._h, ctypes.byref(n)))

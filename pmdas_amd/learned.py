@@ -71,12 +71,16 @@ def heldout_loss(m, X, Y, n):
     return float(np.mean((y - Yh) ** 2)), float(np.var(Yh))
 
 
-def render(scene, W, H, spp, dim, mlp=None, b=1, image_label=False, alpha=0.25, kappa=1.0):
-    """MDAS image (H, W, 4), samples, sampling ms (device) for one scene."""
+def render(scene, W, H, spp, dim, mlp=None, b=1, image_label=False, alpha=0.25, kappa=1.0,
+           kappa_y=None):
+    """MDAS image (H, W, 4), samples, sampling ms (device) for one scene.
+    kappa_y (optional) separates the network's exponent from kappa."""
     g, target = make_grid(W, H, spp, dim, b, alpha=alpha)
     if mlp is not None:
         g.set_learned(_lib.LEARN_SCORE | (_lib.LEARN_IMAGE_LABEL if image_label else 0), mlp)
-        if kappa != 1.0:
+        if kappa_y is not None:
+            g.set_learned_exponents(0.0 if image_label else kappa, kappa_y)
+        elif kappa != 1.0:
             g.set_learned_exponent(kappa)
     g.set_timing(True)
     g.Run(scene, target)
