@@ -132,6 +132,8 @@ def run_batch(steps, log=print):
         targets.append(target)
         bufs.append(hip.DeviceBuffer(w["W"] * w["H"] * 16))
     batch = kdgrid.Batch(grids, scenes)
+    shape = batch.autotune(targets)
+    log(f"[bench] batch of {len(imgs)}: autotuned launch shape {shape} workgroups")
     times = []
     for i in range(steps + 1):
         hip.synchronize()
@@ -151,7 +153,7 @@ def run_batch(steps, log=print):
     ms = sum(times) / len(times)
     res = dict(images=len(imgs), pixels=sum(w["W"] * w["H"] for w in imgs), samples=n,
                ms_per_step=round(ms, 2), e2e_msamples_per_s=round(n / (ms * 1e3), 1),
-               passes_max=max(g.pass_stats().shape[0] for g in grids),
+               passes_max=max(g.pass_stats().shape[0] for g in grids), launch_shape=shape,
                note="wall time of batch run (with the integrand) + 32 reconstructions")
     batch.close()
     for b in bufs:

@@ -200,6 +200,11 @@ int mdas_batch_destroy(mdas_batch_handle b);
 int mdas_batch_run(mdas_batch_handle b, const long long* targets, void* stream, int use_graph);
 /* wall time of the last run (ms), blocks of passes run, dimension groups */
 int mdas_batch_info(mdas_batch_handle b, double* wall_ms, int* pass_blocks, int* groups);
+/* Runs the batch (with graph replay) once per candidate launch shape --
+ * 2048 .. 32768 workgroups per launch in all, shared by a dimension group's
+ * images -- and keeps the fastest; *chosen receives it.  The images' results
+ * do not depend on the shape. */
+int mdas_batch_autotune(mdas_batch_handle b, const long long* targets, void* stream, int* chosen);
 
 /* ---- configuration files ----------------------------------------------------
  * Reader of the reference's ".env" files (SDK/sutil/Environment.cpp:593-792)

@@ -144,6 +144,7 @@ SIGNATURES = {
     "mdas_batch_run": (_I, [_H, _P, _P, _I]),
     "mdas_batch_info": (_I, [_H, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I),
                              ctypes.POINTER(_I)]),
+    "mdas_batch_autotune": (_I, [_H, _P, _P, ctypes.POINTER(_I)]),
     "mdas_env_read": (_I, [ctypes.c_char_p, ctypes.POINTER(MdasEnv)]),
     "mdas_env_last_error": (ctypes.c_char_p, []),
     "mdas_env_config": (_I, [ctypes.POINTER(MdasEnv), _LL, ctypes.POINTER(MdasConfig)]),

@@ -55,6 +55,7 @@ struct mdas_batch {
     int passes_per_block = 4;
     double last_ms = 0.0;
     int last_blocks = 0;
+    int shape = 8192;  // workgroups per launch in all (mdas_batch_autotune)
 };
 
 extern "C" {
@@ -222,6 +223,39 @@ int mdas_batch_run(mdas_batch_handle b, const long long* targets, void* stream, 
     }
     b->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     b->last_blocks = blocks_run;
+    return MDAS_OK;
+}
+
+// The launch shape of a batch: workgroups per image for each dimension group
+// (per launch about `total` in all, clamped to [64, kChunkBlocks] per image).
+static void set_batch_shape(mdas_batch* b, int total) {
+    for (Group& g : b->groups) {
+        const int m = (int)g.members.size();
+        g.blocks = std::max(64, std::min(mdas::kChunkBlocks, total / m));
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+        g.exec = nullptr;
+        g.graph = nullptr;
+    }
+}
+
+int mdas_batch_autotune(mdas_batch_handle b, const long long* targets, void* stream, int* chosen) {
+    if (!b || !targets) return mdas::set_error(MDAS_ERR_INVALID, "null argument");
+    static const int cand[] = {2048, 4096, 8192, 16384, 32768};
+    int best = cand[0];
+    double best_ms = 0.0;
+    for (int c : cand) {
+        set_batch_shape(b, c);
+        const int rc = mdas_batch_run(b, targets, stream, 1);
+        if (rc) return rc;
+        if (best_ms == 0.0 || b->last_ms < best_ms) {
+            best_ms = b->last_ms;
+            best = c;
+        }
+    }
+    set_batch_shape(b, best);
+    b->shape = best;
+    if (chosen) *chosen = best;
     return MDAS_OK;
 }
 

@@ -81,6 +81,15 @@ class Batch:
         check(_lib.load().mdas_batch_run(self._h, t, stream.handle if stream else None,
                                          1 if use_graph else 0))
 
+    def autotune(self, targets, stream=None):
+        """Launch shape (workgroups per launch) of the fastest of the
+        candidate runs; the batch keeps it."""
+        t = (ctypes.c_longlong * len(targets))(*[int(x) for x in targets])
+        c = ctypes.c_int(0)
+        check(_lib.load().mdas_batch_autotune(self._h, t, stream.handle if stream else None,
+                                              ctypes.byref(c)))
+        return c.value
+
     def info(self):
         ms, blocks, groups = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
         check(_lib.load().mdas_batch_info(self._h, ctypes.byref(ms), ctypes.byref(blocks),

@@ -122,3 +122,22 @@ def test_batch_learned_equals_single_runs(kd):
             solo.close()
     batch.close()
     net.close()
+
+
+def test_batch_autotune_keeps_results(kd):
+    """The autotuner's candidate launch shapes all give the solo results; the
+    chosen shape is one of the candidates."""
+    made = [_make(kd, s) for s in IMAGES[:3]]
+    batch = kd.Batch([m[0] for m in made], [m[1] for m in made])
+    targets = [m[2] for m in made]
+    shape = batch.autotune(targets)
+    assert shape in (2048, 4096, 8192, 16384, 32768)
+    batch.run(targets, use_graph=True)
+    for g, sc, target, args in made:
+        solo = kd.KDGrid(*args)
+        solo.Run(sc, target)
+        st, ref = g.download_state(), solo.download_state()
+        for k in st:
+            assert np.array_equal(st[k], ref[k]), k
+        solo.close()
+    batch.close()
