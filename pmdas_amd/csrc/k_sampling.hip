@@ -598,7 +598,6 @@ __device__ __forceinline__ void compact_body(const DevGrid& a, int blk, int nblk
     __shared__ int lds[8];
     if (a.cnt->done) return;
     const int L = a.cnt->n_leaves_old;
-    const int N0 = a.cnt->n_samples_old;
     int beg, end;
     chunk_range(L, nblk, blk, beg, end);
     int2 base = a.part_off[blk];

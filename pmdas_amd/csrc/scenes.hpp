@@ -366,7 +366,7 @@ inline void make_scene(Params* P, int kind, uint32_t seed, int dim, float scale_
     P->lens_radius = 0.05f;
     P->focal_distance = 5.0f;
     // light direction normalize(-0.3, 1, -0.5)
-    P->light_dir[0] = -0.26002076f; P->light_dir[1] = 0.86673587f; P->light_dir[2] = -0.43336794f;
+    P->light_dir[0] = -0.259160519f; P->light_dir[1] = 0.863868415f; P->light_dir[2] = -0.431934208f;
     P->light_color[0] = 1.0f; P->light_color[1] = 0.95f; P->light_color[2] = 0.9f;
     P->ambient = 0.1f;
     P->plane_y = -2.0f;
