@@ -48,6 +48,9 @@ def test_pool_env_runs(tmp_path):
     assert log["TOTAL SAMPLES"][0] >= 4 * 1024 * 1024
     img = _read_pfm(out + ".pfm")
     assert img.shape == (1024, 1024, 3) and np.isfinite(img).all()
+    from pmdas_amd import metrics
+    exr = metrics.read_exr(out + ".exr")  # rows bottom to top, like the PFM
+    assert np.array_equal(exr[..., :3], img) and (exr[..., 3] == 1).all()
     dens = _read_pfm(out + "-density.pfm")
     assert dens.shape == (1024, 1024, 3) and dens.max() > 0
     assert open(out + ".ppm", "rb").read(15).startswith(b"P6\n1024 1024\n")
