@@ -50,6 +50,34 @@ def test_null_arguments_are_errors():
     assert lib.mdas_destroy(None) == 0
 
 
+def test_every_handle_entry_point_rejects_null():
+    """A NULL handle (all other arguments zero / NULL) is an error status, a
+    zero getter value or a no-op destroy -- never a device call or a crash."""
+    from pmdas_amd import _lib
+    lib = _lib.load()
+    H = _lib.SIGNATURES["mdas_destroy"][1][0]
+    seen = 0
+    for name, (res, args) in _lib.SIGNATURES.items():
+        if not args or args[0] is not H:
+            continue
+        vals = [0 if a in (ctypes.c_int, ctypes.c_longlong, ctypes.c_uint32) else
+                0.0 if a is ctypes.c_float else None for a in args]
+        r = getattr(lib, name)(*vals)
+        seen += 1
+        if name.endswith("_destroy"):
+            assert r == 0, name
+        elif name.startswith("mdas_get_") and res is not ctypes.c_int or name in (
+                "mdas_get_dim", "mdas_get_candidates_num", "mdas_get_max_node_size",
+                "mdas_get_new_samples", "mdas_get_new_nodes", "mdas_get_number_of_samples",
+                "mdas_get_number_of_nodes", "mdas_get_max_samples", "mdas_get_passes",
+                "mdas_scene_params"):
+            assert r in (0, 0.0, None), name
+            assert lib.mdas_last_error(), name
+        else:
+            assert r == _lib.MDAS_ERR_INVALID, (name, r)
+    assert seen > 50
+
+
 def test_product_package_does_not_reference_the_oracle():
     pkg = os.path.join(ROOT, "pmdas_amd")
     for dirpath, _, files in os.walk(pkg):
