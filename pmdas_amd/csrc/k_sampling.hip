@@ -48,13 +48,16 @@ __device__ __forceinline__ void initial_body(const DevGrid& a, int L0, int blk, 
         }
         __syncthreads();
         // leaf i's samples are 4i..4i+3, so the per-leaf point copies of the
-        // workgroup's leaves are the same contiguous range
+        // workgroup's leaves hold the same floats, one padded row per leaf
         const int nl = min(256, L0 - base);
         float* out = a.coords + (size_t)base * 4 * D;
-        float* outp = a.pts + (size_t)base * 4 * D;
-        for (int k = threadIdx.x; k < nl * 4 * D; k += 256) {
-            out[k] = stage[k];
-            outp[k] = stage[k];
+        float* outp = a.pts + (size_t)base * pts_stride(D);
+        for (int k = threadIdx.x; k < nl * 4 * D; k += 256) out[k] = stage[k];
+        // whole padded rows, so that the stores cover full lines
+        constexpr int PS = pts_stride(D);
+        for (int k = threadIdx.x; k < nl * PS; k += 256) {
+            const int l = k / PS, r = k - l * PS;
+            outp[k] = r < 4 * D ? stage[l * 4 * D + r] : 0.0f;
         }
         __syncthreads();
     }
@@ -305,8 +308,8 @@ __device__ __forceinline__ void select_body(const DevGrid& a, int blk, int nblk)
             const bool split = sel && cnt == 4u;
             a.flags[i] = (unsigned char)(cnt | (sel ? 8u : 0u) | (split ? 16u : 0u));
             // a selected leaf's state after the placement's candidatesNum * D
-            // draws, by the LCG's affine jump-ahead; the placement recovers
-            // the state it starts from with the inverse map
+            // draws, by the LCG's affine jump-ahead; the compaction recovers
+            // the state the placement starts from with the inverse map
             if (sel) s = a.jmul * s + a.jadd;
             a.seed[i] = s;
             ns += sel ? 1 : 0;
@@ -428,14 +431,13 @@ __global__ __launch_bounds__(1024) void k_scan_counts_b(const DevGrid* grids, in
 // kdgrid.cu:328-477 for one selected leaf: best candidate among
 // candidatesNum uniform points of the box (max over candidates of the min
 // Euclidean distance to the leaf's samples, earliest wins ties), appended at
-// sample index k; a full leaf is split at the median of its 5 samples along
-// the largest box axis, keeping the left half in place and the right half
-// at leaf index m.
+// sample index k, the draws starting from LCG state s; a full leaf is split
+// at the median of its 5 samples along the largest box axis, keeping the left
+// half in place and the right half at leaf index m.
 template <int D>
 __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m, unsigned cnt,
-                                           int L) {
+                                           int L, unsigned s) {
     constexpr int Q = leaf_quads(D);
-    unsigned s = a.jinv * (a.seed[i] - a.jadd);
     int4 id4;
     float mn[D], mx[D], diag[D];
     load_leaf<D>(a, i, id4, mn, mx);
@@ -444,17 +446,26 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
     const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
     // the leaf's points from its contiguous copy (slot order) instead of
     // gathers at the sample indices
+    // (the row is read whole, as D float4s; the floats of empty slots are
+    // replaced, never used in arithmetic)
     float p[4][D];
-    float* lp = a.pts + (size_t)i * 4 * D;
+    float* lp = a.pts + (size_t)i * pts_stride(D);
+    float row[4 * D];
+    {
+        const float4* r4 = reinterpret_cast<const float4*>(lp);
+#pragma unroll
+        for (int v = 0; v < D; ++v) {
+            const float4 t = r4[v];
+            row[4 * v] = t.x;
+            row[4 * v + 1] = t.y;
+            row[4 * v + 2] = t.z;
+            row[4 * v + 3] = t.w;
+        }
+    }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        if (ids[q] >= 0) {
 #pragma unroll
-            for (int d = 0; d < D; ++d) p[q][d] = lp[q * D + d];
-        } else {
-#pragma unroll
-            for (int d = 0; d < D; ++d) p[q][d] = 0.0f;
-        }
+        for (int d = 0; d < D; ++d) p[q][d] = ids[q] >= 0 ? row[q * D + d] : 0.0f;
     }
     float maxDist = -1.0f;
     float best[D];
@@ -561,7 +572,7 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
         store_leaf<D>(a, i, make_int4(v[0], v[1], E, E), mn, cmx);
         store_leaf<D>(a, m, make_int4(v[2], v[3], v[4], E), cmn, mx);
         // point copies: sorted positions 0, 1 -> left child, 2..4 -> right
-        float* rp = a.pts + (size_t)m * 4 * D;
+        float* rp = a.pts + (size_t)m * pts_stride(D);
 #pragma unroll
         for (int j = 0; j < 5; ++j) {
             float pt[D];
@@ -601,13 +612,14 @@ __device__ __forceinline__ void compact_body(const DevGrid& a, int blk, int nblk
     int beg, end;
     chunk_range(L, nblk, blk, beg, end);
     int2 base = a.part_off[blk];
-    constexpr int U = 4;  // tiles whose flags are loaded together
+    constexpr int U = 4;  // tiles whose flags and states are loaded together
     for (int t00 = beg; t00 < end; t00 += U * 256) {
-        unsigned fv[U];
+        unsigned fv[U], sv[U];
 #pragma unroll
         for (int r = 0; r < U; ++r) {
             const int i = t00 + r * 256 + threadIdx.x;
             fv[r] = i < end ? a.flags[i] : 0u;
+            sv[r] = i < end ? a.seed[i] : 0u;
         }
 #pragma unroll
         for (int r = 0; r < U; ++r) {
@@ -619,9 +631,13 @@ __device__ __forceinline__ void compact_body(const DevGrid& a, int blk, int nblk
             const unsigned cnt = f & 7u;
             int rs, rl, ts, tl;
             block_rank2(sel, split, lds, rs, rl, ts, tl);
-            if (sel)  // the new sample is N0 + list index; a split leaf holds 4
+            if (sel) {  // the new sample is N0 + list index; a split leaf holds 4
                 a.qlist[base.x + rs] =
                     make_int2(i | (int)((cnt & 3u) << 30), split ? L + base.y + rl : -1);
+                // the selection stored the state after the candidate draws;
+                // the placement starts from the state before them
+                a.qseed[base.x + rs] = a.jinv * (sv[r] - a.jadd);
+            }
             if (i < end) a.flags[i] = (unsigned char)(split ? 2u : (sel ? cnt + 1u : cnt));
             base.x += ts;
             base.y += tl;
@@ -644,7 +660,7 @@ __device__ __forceinline__ void place_body(const DevGrid& a, int blk, int nblk) 
     for (int j = blk * 256 + threadIdx.x; j < n; j += nblk * 256) {
         const int2 q = a.qlist[j];
         const unsigned cnt = q.y >= 0 ? 4u : ((unsigned)q.x >> 30);
-        place_leaf<D>(a, q.x & 0x3fffffff, N0 + j, q.y, cnt, L);
+        place_leaf<D>(a, q.x & 0x3fffffff, N0 + j, q.y, cnt, L, a.qseed[j]);
     }
 }
 
@@ -664,7 +680,7 @@ __global__ __launch_bounds__(256) void k_build_pts(DevGrid a, int L) {
     for (int i = blockIdx.x * 256 + threadIdx.x; i < L; i += gridDim.x * 256) {
         const int4 id = load_slots(a, i, Q);
         const int ids[4] = {id.x, id.y, id.z, id.w};
-        float* lp = a.pts + (size_t)i * 4 * D;
+        float* lp = a.pts + (size_t)i * pts_stride(D);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             if (ids[q] >= 0 && ids[q] < a.cap_s) {

@@ -54,13 +54,19 @@ static const int kMaxPassStats = 4096;
 // cache-line segment.
 __host__ __device__ constexpr int leaf_quads(int D) { return 1 + (2 * D + 3) / 4; }
 
+// Floats per row of the per-leaf point copies: the 4 * D floats of the four
+// slots, padded to 64 bytes (D = 3, 4) or 128 bytes (D = 5, 6) so that a row
+// read or written at a random leaf stays inside one aligned block.
+__host__ __device__ constexpr int pts_stride(int D) { return 4 * D <= 16 ? 16 : 32; }
+
 // Plain-pointer view of the device state passed to kernels by value.
 struct DevGrid {
     float* coords;         // [cap_s * D]  AoS (the integrand contract)
     float* values;         // [cap_s * 3]  float3 AoS
     float4* leaf;          // [cap_l * leaf_quads(D)]  slots (~D = empty) + box
-    float* pts;            // [cap_l * 4 * D]  copy of each leaf's sample points by slot
+    float* pts;            // [cap_l * pts_stride(D)]  copy of each leaf's sample points by slot
     int2* qlist;           // [cap_l]      placement list (leaf | count << 30, right child)
+    unsigned* qseed;       // [cap_l]      placement list: LCG state before the leaf's candidate draws
     float* err;            // [cap_l]      < 0 = stale
     unsigned* seed;        // [cap_l]      LCG state, 0 = unset
     unsigned char* flags;  // [cap_l]      bits 0-2 count, 3 selected, 4 split
