@@ -7,8 +7,8 @@
 //  2. k_scan_keys / k_recon_emit: every leaf emits one 64-bit key per tile,
 //         key = morton2(tile_x, tile_y) << 32 | leaf index,
 //     in leaf order (a workgroup-chunked exclusive scan gives the offsets).
-//  3. radix_sort_keys: stable LSD radix sort (8-bit digits, LDS-staged
-//     scatter) on the Morton bits only; since keys are emitted in leaf order
+//  3. radix_sort_keys: stable LSD radix sort (digits of up to 10 bits,
+//     LDS-staged scatter) on the Morton bits only; since keys are emitted in leaf order
 //     the result is the total 64-bit order.
 //  4. k_tile_ranges: start / end of every tile's key segment.
 //  5. k_gather: a group of 16 or 64 lanes owns a tile of 16 or 64 pixels,
@@ -25,6 +25,7 @@ namespace mdas {
 
 static const int kSortBlocks = 1024;
 static const int kSortMaxBits = 10;  // digit width limit (histograms of 1024 bins)
+static const int kSortKeysPerThread = 16;  // scatter tile: 4096 keys staged in LDS (8192: slower)
 
 struct ReconGeo {
     int W, H;
@@ -217,9 +218,10 @@ __global__ __launch_bounds__(1024) void k_scan_down(unsigned* data, int n, const
 // pass, BITS chosen per sort so the Morton bits take as few passes as
 // possible (at most 10 bits a pass).  Reduce-then-scan: per-workgroup digit
 // histograms, one exclusive scan over (digit, workgroup), then a scatter in
-// which every workgroup ranks a tile of 256 keys by digit with wave ballots,
-// stages the tile in LDS in digit order and writes each digit run
-// contiguously.
+// which every workgroup ranks a tile of 4096 keys by digit with wave ballots
+// (16 sub-tiles of 256), stages the tile in LDS in digit order and writes each
+// digit run contiguously -- runs of 8 keys on average at 512 digits, where a
+// 256-key tile would leave most keys alone in their line.
 template <int BITS>
 __global__ __launch_bounds__(256) void k_rs_hist(const unsigned long long* keys, const int* n_ptr,
                                                  int shift, unsigned* hist) {
@@ -242,41 +244,62 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const unsigned long long* in
                                                     int shift, const unsigned* hist) {
     constexpr int NB = 1 << BITS;
     constexpr int R = NB > 256 ? NB / 256 : 1;  // digits per thread in the tile scan
+    constexpr int KPT = kSortKeysPerThread;
+    constexpr int TILE = 256 * KPT;
     __shared__ unsigned base[NB];
     __shared__ int wcnt[4][NB];
+    __shared__ int run[NB];   // keys of each digit in the tile's earlier sub-tiles
     __shared__ int toff[NB];
-    __shared__ int ttot[NB];
     __shared__ int scr[4];
-    __shared__ unsigned long long stage[256];
+    __shared__ unsigned long long stage[TILE];
     const int n = *n_ptr;
     const int w = threadIdx.x >> 6;
     for (int d = threadIdx.x; d < NB; d += 256) base[d] = hist[d * gridDim.x + blockIdx.x];
     int beg, end;
     chunk_range(n, gridDim.x, blockIdx.x, beg, end);
-    for (int t0 = beg; t0 < end; t0 += 256) {
-        const int i = t0 + threadIdx.x;
-        const bool valid = i < end;
-        const unsigned long long k = valid ? in[i] : 0ull;
-        const unsigned d = (unsigned)(k >> shift) & (unsigned)(NB - 1);
-        for (int q = 0; q < 4; ++q)
-            for (int dd = threadIdx.x; dd < NB; dd += 256) wcnt[q][dd] = 0;
-        __syncthreads();
-        unsigned long long m = __ballot(valid);
+    for (int t0 = beg; t0 < end; t0 += TILE) {
+        // sub-tile s holds keys t0 + 256 s + thread; their loads go out together
+        unsigned long long k[KPT];
 #pragma unroll
-        for (int b = 0; b < BITS; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const unsigned long long bal = __ballot(bit);
-            m &= bit ? bal : ~bal;
+        for (int s = 0; s < KPT; ++s) {
+            const int i = t0 + s * 256 + threadIdx.x;
+            k[s] = i < end ? in[i] : 0ull;
         }
-        const int rw = lanes_below(m);
-        if (valid && rw == 0) wcnt[w][d] = __popcll(m);
+        for (int d = threadIdx.x; d < NB; d += 256) run[d] = 0;
+        // rank of each key among the tile's keys of its digit, in input
+        // order: earlier sub-tiles, then lower waves, then lower lanes
+        int rk[KPT];
+#pragma unroll
+        for (int s = 0; s < KPT; ++s) {
+            const bool valid = t0 + s * 256 + (int)threadIdx.x < end;
+            const unsigned d = (unsigned)(k[s] >> shift) & (unsigned)(NB - 1);
+            for (int q = 0; q < 4; ++q)
+                for (int dd = threadIdx.x; dd < NB; dd += 256) wcnt[q][dd] = 0;
+            __syncthreads();
+            unsigned long long m = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < BITS; ++b) {
+                const bool bit = (d >> b) & 1u;
+                const unsigned long long bal = __ballot(bit);
+                m &= bit ? bal : ~bal;
+            }
+            const int rw = lanes_below(m);
+            if (valid && rw == 0) wcnt[w][d] = __popcll(m);
+            __syncthreads();
+            int pre = run[d];
+            for (int q = 0; q < w; ++q) pre += wcnt[q][d];
+            rk[s] = pre + rw;
+            __syncthreads();
+            for (int dd = threadIdx.x; dd < NB; dd += 256)
+                run[dd] += wcnt[0][dd] + wcnt[1][dd] + wcnt[2][dd] + wcnt[3][dd];
+        }
         __syncthreads();
         {
             int loc[R], sum = 0;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int dd = (int)threadIdx.x * R + r;
-                loc[r] = dd < NB ? wcnt[0][dd] + wcnt[1][dd] + wcnt[2][dd] + wcnt[3][dd] : 0;
+                loc[r] = dd < NB ? run[dd] : 0;
                 sum += loc[r];
             }
             int all;
@@ -286,26 +309,28 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const unsigned long long* in
                 const int dd = (int)threadIdx.x * R + r;
                 if (dd < NB) {
                     toff[dd] = ex;
-                    ttot[dd] = loc[r];
                     ex += loc[r];
                 }
             }
         }
         __syncthreads();
-        const int nvalid = min(256, end - t0);
-        if (valid) {
-            int pre = 0;
-            for (int q = 0; q < w; ++q) pre += wcnt[q][d];
-            stage[toff[d] + pre + rw] = k;
+#pragma unroll
+        for (int s = 0; s < KPT; ++s) {
+            if (t0 + s * 256 + (int)threadIdx.x < end) {
+                const unsigned d = (unsigned)(k[s] >> shift) & (unsigned)(NB - 1);
+                stage[toff[d] + rk[s]] = k[s];
+            }
         }
         __syncthreads();
-        if ((int)threadIdx.x < nvalid) {
-            const unsigned long long kk = stage[threadIdx.x];
+        // the tile in digit order: each digit's run goes to its own range
+        const int nvalid = min(TILE, end - t0);
+        for (int p = threadIdx.x; p < nvalid; p += 256) {
+            const unsigned long long kk = stage[p];
             const unsigned dd = (unsigned)(kk >> shift) & (unsigned)(NB - 1);
-            out[base[dd] + (threadIdx.x - toff[dd])] = kk;
+            out[base[dd] + (p - toff[dd])] = kk;
         }
         __syncthreads();
-        for (int dd = threadIdx.x; dd < NB; dd += 256) base[dd] += ttot[dd];
+        for (int dd = threadIdx.x; dd < NB; dd += 256) base[dd] += run[dd];
         __syncthreads();
     }
 }
