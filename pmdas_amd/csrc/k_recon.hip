@@ -17,6 +17,8 @@
 //     reference's own expressions, including its accumulated pixel bounds
 //     (pMax += step), so a pixel is the sum of its contributions in leaf
 //     index order.
+#include <type_traits>
+
 #include "mdas_leaf.hpp"
 
 #pragma clang fp contract(off)
@@ -475,9 +477,22 @@ __global__ __launch_bounds__(256) void k_gather(DevGrid a, ReconGeo G,
             c1 = a.rrec[(size_t)i * 2 + 1];
         }
         knext = gl < nrec_of(rd + 2) ? keys[s0 + (rd + 2) * GS + gl] : 0ull;
+        // the round's records that cover this lane's pixel, then only those,
+        // in index order (the others add nothing): the loop runs as often as
+        // the most-covered pixel of the wave needs, not once per record
+        using CovT = typename std::conditional<(GS > 32), unsigned long long, unsigned>::type;
+        CovT cov = 0;
         for (int j = 0; j < nrec; ++j) {
             const float4 q2 = r2[w][gbase + j];
-            if (GS <= 32 ? ((__float_as_uint(q2.w) >> gl) & 1u) != 0u : inside_span(q2, px, py)) {
+            const bool in = GS <= 32 ? ((__float_as_uint(q2.w) >> gl) & 1u) != 0u
+                                     : inside_span(q2, px, py);
+            cov |= (CovT)(in ? 1u : 0u) << j;
+        }
+        while (cov != 0) {
+            const int j = GS > 32 ? __builtin_ctzll((unsigned long long)cov) : __builtin_ctz((unsigned)cov);
+            cov &= cov - 1;
+            {
+                const float4 q2 = r2[w][gbase + j];
                 const unsigned spx = __float_as_uint(q2.y), spy = __float_as_uint(q2.z);
                 const int x0 = (int)(spx & 0xffffu), y0 = (int)(spy & 0xffffu);
                 const float4 q0 = r0[w][gbase + j];
