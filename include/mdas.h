@@ -159,7 +159,7 @@ int mdas_kernel_stats(mdas_handle h, int index, char* name, int name_cap, int* l
                       double* total_ms, double* bytes);
 
 /* Tile of the reconstruction gather: tile_w x tile_h pixels, 16 or 64 in
- * all (default 4 x 4).  The result does not depend on it. */
+ * all (default 8 x 8).  The result does not depend on it. */
 int mdas_set_recon_tile(mdas_handle h, int tile_w, int tile_h);
 /* Workgroups of the per-pass kernels: leaf chunks of the selection,
  * compaction and reconstruction kernels (64..32768, default 4096) and of

@@ -107,7 +107,7 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
     __shared__ float xs[4][16 * kMlpIn];
     __shared__ float acts[4][16 * kMlpH];
     __shared__ float ys[4][16];
-    if (a.cnt->done) return;
+    if (pass_stopped(a.cnt)) return;
     const bool first = a.cnt->passes == 0;
     const int n_items = first ? a.cnt->n_leaves : a.cnt->new_nodes;
     const int L_old = a.cnt->n_leaves_old;
@@ -247,6 +247,10 @@ __device__ __forceinline__ void err_reduce_learned_body(const DevGrid& a, const 
     __shared__ unsigned sm[16];
     __shared__ int ss[16], sv[16];
     if (a.cnt->done) return;
+    if (a.cnt->n_samples >= a.cnt->limit) {  // the loop stops before this pass
+        if (threadIdx.x == 0) a.cnt->done = 1;
+        return;
+    }
     unsigned mx = 0;
     int st = 0, va = 0;
     for (int k = threadIdx.x; k < G; k += 1024) {

@@ -97,24 +97,14 @@ __device__ __forceinline__ void initial_counters_body(DevCounters* c, int L0) {
 __global__ void k_initial_counters(DevCounters* c, int L0) { initial_counters_body(c, L0); }
 
 // ComputeErrors prologue (kdgrid.cu:684-685): g_error = 0.  A standalone call
-// also re-arms the pass kernels.
+// also re-arms the pass kernels and lifts the sampling loop's limit.
 __global__ void k_reset_errors(DevCounters* c, int standalone) {
-    if (standalone) c->done = 0;
+    if (standalone) {
+        c->done = 0;
+        c->limit = 0x7fffffff;
+    }
     c->gmax_bits = 0;
 }
-
-// One pass of the device-side loop (optixMotionBlur.cpp:781): stop when
-// N >= min(target, max_samples).  gmax starts at the max error of the leaves
-// the previous pass did not split (their errors are not re-estimated).
-__device__ __forceinline__ void pass_begin_body(DevCounters* c) {
-    if (c->done) return;
-    if (c->n_samples >= c->limit) {
-        c->done = 1;
-        return;
-    }
-    c->gmax_bits = c->passes == 0 ? 0u : c->carry_bits;
-}
-__global__ void k_pass_begin(DevCounters* c) { pass_begin_body(c); }
 
 __global__ void k_set_limit(DevCounters* c, int limit) { c->limit = limit; }
 
@@ -126,10 +116,6 @@ __global__ void k_initial_counters_b(const DevGrid* grids, int n, const int* lim
     DevCounters* c = a.cnt;
     initial_counters_body(c, 1 << (a.b * a.D + 2 * a.e));
     c->limit = limits[t];
-}
-__global__ void k_pass_begin_b(const DevGrid* grids, int n) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) pass_begin_body(grids[t].cnt);
 }
 
 // Re-arms the pass kernels for a standalone AdaptiveSampling call.
@@ -187,13 +173,13 @@ __global__ __launch_bounds__(256) void k_errors(DevGrid a) {
 // The same result inside the sampling loop without the sweep over all
 // leaves: the only leaves with error < 0 are the two children of every leaf
 // the last pass split (split_list[t] and L_old + t); the max over the other
-// leaves was carried by the selection kernel into gmax by k_pass_begin.  The
-// first pass after the initial grid re-estimates every leaf.
+// leaves was carried by the selection kernel into gmax by the allocation
+// scan.  The first pass after the initial grid re-estimates every leaf.
 template <int D>
 __device__ __forceinline__ void errors_stale_body(const DevGrid& a, int blk, int nblk) {
     __shared__ float wmax[4];
     __shared__ int wst[8];
-    if (a.cnt->done) return;
+    if (pass_stopped(a.cnt)) return;
     const bool first = a.cnt->passes == 0;
     const int nn = first ? a.cnt->n_leaves : a.cnt->new_nodes;
     const int n_items = first ? nn : 2 * nn;
@@ -226,6 +212,10 @@ __device__ __forceinline__ void err_reduce_body(const DevGrid& a, int G) {
     __shared__ unsigned sm[16];
     __shared__ int ss[16], sv[16];
     if (a.cnt->done) return;
+    if (a.cnt->n_samples >= a.cnt->limit) {  // the loop stops before this pass
+        if (threadIdx.x == 0) a.cnt->done = 1;
+        return;
+    }
     unsigned m = 0;
     int st = 0, va = 0;
     for (int k = threadIdx.x; k < G; k += 1024) {
@@ -413,6 +403,7 @@ __device__ __forceinline__ void scan_counts_body(const DevGrid& a, int G) {
             cn->new_samples = ts;
             cn->new_nodes = tl;
             cn->carry_bits = tm;
+            cn->gmax_bits = tm;  // the next pass's max starts at the carried one
             const int p = cn->passes;
             if (p < kMaxPassStats) a.pstats[p] = PassStat{L, N, ts, tl, tc, cn->stale, cn->stale_vals, 0};
             cn->passes = p + 1;
@@ -728,11 +719,6 @@ hipError_t launch_reset_errors(Grid& g, int standalone, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_pass_begin(Grid& g, hipStream_t st) {
-    hipLaunchKernelGGL(k_pass_begin, dim3(1), dim3(1), 0, st, g.dev.cnt);
-    return hipGetLastError();
-}
-
 hipError_t launch_set_limit(Grid& g, int limit, hipStream_t st) {
     hipLaunchKernelGGL(k_set_limit, dim3(1), dim3(1), 0, st, g.dev.cnt, limit);
     return hipGetLastError();
@@ -807,7 +793,6 @@ hipError_t launch_initial_b(int D, const DevGrid* grids, int n, int blocks, cons
 
 hipError_t launch_pass_b(int D, const DevGrid* grids, int n, int blocks, hipStream_t st,
                          const LearnDev* learns) {
-    hipLaunchKernelGGL(k_pass_begin_b, dim3((n + 63) / 64), dim3(64), 0, st, grids, n);
     const int eb = blocks < 1024 ? blocks : 1024;
     if (learns) {
         const hipError_t e = launch_errors_learned_b(D, grids, learns, n, blocks, st);

@@ -687,7 +687,6 @@ int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream)
     for (;;) {
         for (int b = 0; b < B; ++b) {
             mdas::kbeg(g, st, "k_errors");
-            HIP_TRY(mdas::launch_pass_begin(g, st));
             HIP_TRY(g.learn_mode ? mdas::launch_errors_learned(g, st)
                                  : mdas::launch_errors_stale(g, st));
             mdas::kend(g, st);

@@ -133,7 +133,7 @@ struct Grid {
     int* tile_end = nullptr;
     long long tiles_cap = 0;
     unsigned* rs_hist = nullptr;  // radix histogram [1024 * kSortBlocks] + scan sums
-    int tile_w = 4, tile_h = 4;  // reconstruction tile (mdas_set_recon_tile / autotune)
+    int tile_w = 8, tile_h = 8;  // reconstruction tile (mdas_set_recon_tile / autotune)
     long long last_n_keys = 0;
     long long rrec_cap = 0;       // leaves the reconstruction records can hold
     int* d_scalar = nullptr;      // small device scratch
@@ -192,7 +192,6 @@ hipError_t launch_adaptive(Grid& g, hipStream_t st);
 hipError_t launch_select(Grid& g, hipStream_t st);
 hipError_t launch_scan(Grid& g, hipStream_t st);
 hipError_t launch_place(Grid& g, hipStream_t st);
-hipError_t launch_pass_begin(Grid& g, hipStream_t st);
 hipError_t launch_set_limit(Grid& g, int limit, hipStream_t st);
 hipError_t launch_arm(Grid& g, hipStream_t st);
 hipError_t launch_build_pts(Grid& g, int n_leaves, hipStream_t st);

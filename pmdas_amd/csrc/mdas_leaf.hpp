@@ -87,6 +87,14 @@ __device__ __forceinline__ void store_leaf(const DevGrid& a, int i, int4 id, con
 // callers that derive more statistics from the same reads.  Occupied slots
 // are always 0..n-1: slots fill in order and a split writes both children
 // from slot 0.
+// Stopping rule of the device-side loop (optixMotionBlur.cpp:781): no
+// further pass once N >= min(target, max_samples).  The error kernels of a
+// pass test it (every workgroup reads the same counters) and the reduction
+// after them raises `done` for the rest of the pass.
+__device__ __forceinline__ bool pass_stopped(const DevCounters* c) {
+    return c->done || c->n_samples >= c->limit;
+}
+
 struct LeafSamples {
     float vx[4], vy[4], vz[4];
     int n;
