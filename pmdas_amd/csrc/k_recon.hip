@@ -401,12 +401,6 @@ __global__ __launch_bounds__(256) void k_tile_ranges(const unsigned long long* k
 }
 
 // ---------------------------------------------------------------- gather --
-__device__ __forceinline__ bool inside_span(const float4& q2, int px, int py) {
-    const unsigned spx = __float_as_uint(q2.y), spy = __float_as_uint(q2.z);
-    return px >= (int)(spx & 0xffffu) && px < (int)(spx >> 16) && py >= (int)(spy & 0xffffu) &&
-           py < (int)(spy >> 16);
-}
-
 // GS = pixels per tile (16 or 64); a wave holds 64/GS tiles.
 template <int GS>
 __global__ __launch_bounds__(256) void k_gather(DevGrid a, ReconGeo G,
@@ -447,6 +441,7 @@ __global__ __launch_bounds__(256) void k_gather(DevGrid a, ReconGeo G,
     unsigned long long knext = gl < nrec_of(1) ? keys[s0 + GS + gl] : 0ull;
     for (int rd = 0; rd < rounds; ++rd) {
         const int nrec = nrec_of(rd);
+        unsigned long long own = 0ull;  // 64-pixel tiles: the pixels this lane's record covers
         if (gl < nrec) {
             // per record, once: span, xy area, the span's first pixel bounds
             // (kdgrid.cu:532-536, the same operations every lane would do)
@@ -459,6 +454,16 @@ __global__ __launch_bounds__(256) void k_gather(DevGrid a, ReconGeo G,
                     const int qx = tx * G.TW + k % G.TW, qy = ty * G.TH + k / G.TW;
                     if (qx >= x0 && qx < x1 && qy >= y0 && qy < y1) mask |= 1u << k;
                 }
+            } else {
+                // the span clipped to the tile is a rectangle of bits: columns
+                // [cx0, cx1) of every row in [cy0, cy1), row r at bits TW r..
+                const int cx0 = min(max(x0 - tx * G.TW, 0), G.TW), cx1 = min(max(x1 - tx * G.TW, 0), G.TW);
+                const int cy0 = min(max(y0 - ty * G.TH, 0), G.TH), cy1 = min(max(y1 - ty * G.TH, 0), G.TH);
+                auto low = [](int n) { return n >= 64 ? ~0ull : (1ull << n) - 1ull; };
+                const unsigned long long rep = G.TW >= 64 ? 1ull : ~0ull / low(G.TW);
+                const unsigned long long cols = low(cx1) & ~low(cx0);
+                const unsigned long long rows = rep & low(G.TW * cy1) & ~low(G.TW * cy0);
+                own = cols * rows;  // no carries: the column field is TW bits wide
             }
             r0[w][gbase + gl] = c0;
             r1[w][gbase + gl] = c1;
@@ -482,11 +487,19 @@ __global__ __launch_bounds__(256) void k_gather(DevGrid a, ReconGeo G,
         // the most-covered pixel of the wave needs, not once per record
         using CovT = typename std::conditional<(GS > 32), unsigned long long, unsigned>::type;
         CovT cov = 0;
-        for (int j = 0; j < nrec; ++j) {
-            const float4 q2 = r2[w][gbase + j];
-            const bool in = GS <= 32 ? ((__float_as_uint(q2.w) >> gl) & 1u) != 0u
-                                     : inside_span(q2, px, py);
-            cov |= (CovT)(in ? 1u : 0u) << j;
+        if constexpr (GS > 32) {
+            // one tile per wave, record j in lane j: the wave's coverage
+            // matrix transposed by ballots, column p delivered to lane p
+#pragma unroll
+            for (int p = 0; p < 64; ++p) {
+                const unsigned long long b = __ballot(((own >> p) & 1ull) != 0ull);
+                cov = lane == p ? (CovT)b : cov;
+            }
+        } else {
+            for (int j = 0; j < nrec; ++j) {
+                const float4 q2 = r2[w][gbase + j];
+                cov |= (CovT)((__float_as_uint(q2.w) >> gl) & 1u) << j;
+            }
         }
         while (cov != 0) {
             const int j = GS > 32 ? __builtin_ctzll((unsigned long long)cov) : __builtin_ctz((unsigned)cov);
