@@ -84,3 +84,45 @@ def test_config_bit_exact(mdas, name, kind, dim, W, H, spp, b, e, seed):
     g.close()
     o.close()
     sc.close()
+
+
+def _digest(st):
+    """Per-array digests of a downloaded state (raw bytes)."""
+    import xxhash
+    return {k: xxhash.xxh3_128_hexdigest(memoryview(np.ascontiguousarray(v)).cast("B"))
+            for k, v in st.items()}
+
+
+def test_batch_of_32_equals_single_runs(mdas):
+    """The benchmark's batch at its real size -- 32 images, pixel counts
+    log-uniform in [256^2, 2048^2], 5D, 32 spp, 746 M samples -- advanced as
+    one batch replayed from a captured hipGraph: every image's final state
+    equals the same image run alone, bit for bit."""
+    import bench
+    imgs = bench.batch_images()
+    grids, scenes, targets = [], [], []
+    for w in imgs:
+        target, cap = bench.capacity(w)
+        grids.append(mdas.KDGrid(w["dim"], cap, w["cand"], w["b"], w["e"], w["alpha"],
+                                 float(w["W"]), float(w["H"])))
+        scenes.append(mdas.Scene(w["kind"], w["seed"], w["dim"], float(w["W"]), float(w["H"])))
+        targets.append(target)
+    _say(f"batch of {len(imgs)}: run")
+    batch = mdas.Batch(grids, scenes)
+    batch.run(targets, use_graph=True)
+    want = []
+    for i, g in enumerate(grids):
+        want.append((g.GetNumberOfSamples(), g.GetNumberOfNodes(), _digest(g.download_state())))
+        assert want[-1][0] >= targets[i]
+    batch.close()
+    for g in grids:
+        g.close()
+    for i, (w, sc, target) in enumerate(zip(imgs, scenes, targets)):
+        _say(f"image {i}: {w['W']}x{w['H']} alone")
+        g = mdas.KDGrid(w["dim"], bench.capacity(w)[1], w["cand"], w["b"], w["e"], w["alpha"],
+                        float(w["W"]), float(w["H"]))
+        g.Run(sc, target)
+        got = (g.GetNumberOfSamples(), g.GetNumberOfNodes(), _digest(g.download_state()))
+        assert got == want[i], (i, w["W"], w["H"])
+        g.close()
+        sc.close()
