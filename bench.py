@@ -242,9 +242,16 @@ def run_frames(log=print):
     e = L.extra_bits(W, H, D, c["b"])
     target = c["spp"] * W * H
     cap = int(target * 1.25) + 4 * (1 << (c["b"] * D + 2 * e)) // 8
-    per_frame = cap * (4 * D + 12) + (cap // 2) * (16 * 4 + 16 * D + 4 + 4 + 1 + 4 + 16 + 48 + 1 + 68)
+    # device bytes of one frame's grid: samples (coordinates, values) and per
+    # leaf the record, the padded point-copy row, placement list entry and
+    # LCG state, error, state, flags, split list, learned features and plain
+    # error, reconstruction record and ~1.2 sort keys (twice)
+    leaf_rec = 16 * (1 + (2 * D + 3) // 4)
+    pts_row = 4 * (16 if 4 * D <= 16 else 32)
+    per_leaf = leaf_rec + pts_row + 8 + 4 + 4 + 4 + 1 + 4 + 4 * 16 + 4 + 32 + 2 * 8 * 1.2
+    per_frame = int(cap * (4 * D + 12) + (cap // 2) * per_leaf)
     free, _ = hip.mem_info()
-    F = int(max(1, min(c["frames"], (free * 0.85) // per_frame)))
+    F = int(max(1, min(c["frames"], (free * 0.9) // per_frame)))
     log(f"[bench] frames: {per_frame / 2**30:.0f} GiB per frame grid, {free / 2**30:.0f} GiB free, "
         f"{F} frame(s) per batch")
     grids = [kdgrid.KDGrid(D, cap, c["cand"], c["b"], e, c["alpha"], float(W), float(H))

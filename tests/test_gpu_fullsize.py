@@ -126,3 +126,57 @@ def test_batch_of_32_equals_single_runs(mdas):
         assert got == want[i], (i, w["W"], w["H"])
         g.close()
         sc.close()
+
+
+def test_4k_6d_ao_frame_at_full_size(mdas):
+    """One frame of the 4K ambient-occlusion workload (3840x2160, 6D, 64 spp,
+    learned scoring on): 530 M samples and 3.2 G coordinates, past 2^31, so
+    every sample and coordinate index must be 64-bit.  Checks the tree
+    invariants (Validate: coordinates inside their leaf boxes, the samples a
+    permutation, volumes summing to the domain), the integrand's values at
+    samples spread over the whole index range against the oracle's, and that
+    the reconstructed image carries the leaves' integral (with scale = pixel
+    grid, sum of pixels = sum over leaves of mean x volume)."""
+    import bench
+    from pmdas_amd import learned
+    c = bench.FRAMES
+    W, H, D = c["W"], c["H"], c["dim"]
+    e = learned.extra_bits(W, H, D, c["b"])
+    target = c["spp"] * W * H
+    cap = int(target * 1.25) + 4 * (1 << (c["b"] * D + 2 * e)) // 8
+    g = mdas.KDGrid(D, cap, c["cand"], c["b"], e, c["alpha"], float(W), float(H))
+    net = mdas.Mlp(3)
+    g.set_learned(2, net)
+    g.set_learned_exponent(0.125)
+    sc = mdas.Scene(c["kind"], c["first_seed"], D, float(W), float(H))
+    _say("4k frame: run")
+    g.Run(sc, target)
+    N, L = g.GetNumberOfSamples(), g.GetNumberOfNodes()
+    assert N >= target and N * D > 2 ** 31
+    _say(f"4k frame: validate ({N} samples, {L} leaves)")
+    assert g.Validate()
+    _say("4k frame: reconstruction")
+    img = g.integrate_image(W, H)
+    assert np.isfinite(img).all() and img[..., :3].min() >= 0.0 and img[..., :3].max() <= 1.0 + 1e-5
+    image_sum = img[..., :3].astype(np.float64).sum(axis=(0, 1))
+    del img
+    _say("4k frame: state")
+    st = g.download_state()
+    g.close()
+    net.close()
+    rng = np.random.default_rng(0)
+    pick = np.unique(np.concatenate([rng.integers(0, N, 100000), np.arange(N - 4096, N)]))
+    want = pyoracle.eval_scene(sc.params_bytes(), st["coords"][pick], D)
+    assert np.array_equal(st["values"][pick].view(np.uint32), want.view(np.uint32))
+    sc.close()
+    leaf_sum = np.zeros(3)
+    for a in range(0, L, 1 << 24):  # sum of leaf mean x volume, in chunks
+        idx = st["idx"][a:a + (1 << 24)]
+        ok = idx >= 0
+        v = st["values"][np.where(ok, idx, 0)]
+        v[~ok] = 0.0
+        mean = v.sum(axis=1, dtype=np.float64) / ok.sum(axis=1)[:, None]
+        box = st["box"][a:a + (1 << 24)].astype(np.float64)
+        vol = np.prod(box[:, D:] - box[:, :D], axis=1)
+        leaf_sum += (mean * vol[:, None]).sum(axis=0)
+    assert np.allclose(image_sum, leaf_sum, rtol=1e-4), (image_sum, leaf_sum)
