@@ -172,7 +172,8 @@ __global__ __launch_bounds__(256) void k_errors(DevGrid a) {
 
 // The same result inside the sampling loop without the sweep over all
 // leaves: the only leaves with error < 0 are the two children of every leaf
-// the last pass split (split_list[t] and L_old + t); the max over the other
+// the last pass split (split_list[t] and L_old + t, whose sample indices and
+// volumes the placement left in the split's entry); the max over the other
 // leaves was carried by the selection kernel into gmax by the allocation
 // scan.  The first pass after the initial grid re-estimates every leaf.
 template <int D>
@@ -181,19 +182,38 @@ __device__ __forceinline__ void errors_stale_body(const DevGrid& a, int blk, int
     __shared__ int wst[8];
     if (pass_stopped(a.cnt)) return;
     const bool first = a.cnt->passes == 0;
-    const int nn = first ? a.cnt->n_leaves : a.cnt->new_nodes;
-    const int n_items = first ? nn : 2 * nn;
-    const int L_old = a.cnt->n_leaves_old;
     float m = 0.0f;
     int n_stale = 0, n_vals = 0;
-    for (int t = blk * 256 + threadIdx.x; t < n_items; t += nblk * 256) {
-        const int i = first ? t : (t < nn ? a.split_list[t] : L_old + (t - nn));
-        int n;
-        const float e = leaf_error<D>(a, i, &n);
-        a.err[i] = e;
-        n_stale += 1;
-        n_vals += n;
-        m = fmaxf(m, e);
+    if (first) {
+        const int L = a.cnt->n_leaves;
+        for (int i = blk * 256 + threadIdx.x; i < L; i += nblk * 256) {
+            int n;
+            const float e = leaf_error<D>(a, i, &n);
+            a.err[i] = e;
+            n_stale += 1;
+            n_vals += n;
+            m = fmaxf(m, e);
+        }
+    } else {
+        // one thread per split: both children from the split's entry
+        // (sample indices and volumes), not from their leaf records
+        const int nn = a.cnt->new_nodes;
+        const int L_old = a.cnt->n_leaves_old;
+        for (int t = blk * 256 + threadIdx.x; t < nn; t += nblk * 256) {
+            const int4 s0 = a.sdat[2 * t], s1 = a.sdat[2 * t + 1];
+            LeafSamples ls;
+            const int idl[4] = {s0.x, s0.y, -1, -1};
+            const float el = leaf_error_ids(a, idl, __int_as_float(s1.y), ls);
+            const int nl = ls.n;
+            const int idr[4] = {s0.z, s0.w, s1.x, -1};
+            const float er = leaf_error_ids(a, idr, __int_as_float(s1.z), ls);
+            a.err[a.split_list[t]] = el;
+            a.err[L_old + t] = er;
+            n_stale += 2;
+            n_vals += nl + ls.n;
+            m = fmaxf(m, el);
+            m = fmaxf(m, er);
+        }
     }
     errors_epilogue(a, m, n_stale, n_vals, wmax, wst, blk);
 }
@@ -562,6 +582,11 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
         }
         store_leaf<D>(a, i, make_int4(v[0], v[1], E, E), mn, cmx);
         store_leaf<D>(a, m, make_int4(v[2], v[3], v[4], E), cmn, mx);
+        // what the next error pass needs of both children, coalesced by
+        // split rank: their sample indices and box volumes
+        a.sdat[2 * (m - L)] = make_int4(v[0], v[1], v[2], v[3]);
+        a.sdat[2 * (m - L) + 1] = make_int4(v[4], __float_as_int(box_volume<D>(mn, cmx)),
+                                            __float_as_int(box_volume<D>(cmn, mx)), 0);
         // point copies: sorted positions 0, 1 -> left child, 2..4 -> right
         float* rp = a.pts + (size_t)m * pts_stride(D);
 #pragma unroll

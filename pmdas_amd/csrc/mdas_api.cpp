@@ -59,7 +59,7 @@ void lcg_jump(unsigned k, unsigned& A, unsigned& C, unsigned& Ainv) {
 void free_grid(Grid& g) {
     mdas::DevGrid& d = g.dev;
     void* ptrs[] = {d.coords,   d.values, d.leaf,     d.err,  d.seed,       d.flags, d.pts, d.qlist, d.qseed,
-                    d.split_list, d.part, d.part_off, d.epart, d.rrec,      d.cnt,
+                    d.split_list, d.sdat, d.part, d.part_off, d.epart, d.rrec,      d.cnt,
                     d.pstats,   g.keys,   g.keys_alt, g.tile_start, g.tile_end, g.rs_hist,
                     g.d_scalar, g.learn.feat, g.learn.perr, g.learn.rows};
     for (void* p : ptrs)
@@ -188,6 +188,7 @@ int mdas_create(const mdas_config* cfg, mdas_handle* out) {
     if (e == hipSuccess) e = dalloc(&d.seed, cl);
     if (e == hipSuccess) e = dalloc(&d.flags, cl);
     if (e == hipSuccess) e = dalloc(&d.split_list, cl);
+    if (e == hipSuccess) e = dalloc(&d.sdat, 2 * (cl / 2 + 1));  // splits per pass <= cap_l / 2
     if (e == hipSuccess) e = dalloc(&d.part, (size_t)mdas::kChunkBlocks);
     if (e == hipSuccess) e = dalloc(&d.part_off, (size_t)mdas::kChunkBlocks);
     if (e == hipSuccess) e = dalloc(&d.epart, (size_t)mdas::kErrBlocks);
@@ -770,8 +771,12 @@ static double kernel_bytes(const Grid& g, const std::string& k) {
         return n * (4.0 * D + 12.0);
     }
     for (const auto& p : g.pstats_host) {
-        if (k == "k_errors")  // stale leaves: record, values, error; half of them via the list
-            b += (double)p.stale * (LB + 4.0 + 2.0) + 12.0 * p.stale_vals;
+        if (k == "k_errors") {
+            if (&p == &g.pstats_host[0])  // first pass: every leaf's record, values, error
+                b += (double)p.stale * (LB + 4.0) + 12.0 * p.stale_vals;
+            else  // per split: list entry, split entry, two errors; the values
+                b += 0.5 * (double)p.stale * (4.0 + 32.0 + 8.0) + 12.0 * p.stale_vals;
+        }
         else if (k == "k_select")  // error, state, flags read; state, flags written
             b += 14.0 * p.L;
         else if (k == "k_scan_counts")
@@ -783,8 +788,8 @@ static double kernel_bytes(const Grid& g, const std::string& k) {
                                                                           // record, new sample
                  4.0 * D * p.selcnt +                                     // the leaf's points
                  (double)(p.sel - p.split) * (16.0 + 4.0 * D) +           // slots + point of an insert
-                 (double)p.split * (2.0 * LB + 20.0 * D + 8.0 + 4.0 + 1.0 + 4.0);  // two records
-                                                                          // and their points, errors
+                 (double)p.split * (2.0 * LB + 20.0 * D + 8.0 + 4.0 + 1.0 + 4.0 + 32.0);  // two
+                                                            // records and their points, errors, split entry
     }
     const double L = (double)g.host.n_leaves, N = (double)g.host.n_samples;
     const double K = (double)g.last_n_keys;

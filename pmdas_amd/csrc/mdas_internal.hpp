@@ -71,6 +71,8 @@ struct DevGrid {
     unsigned* seed;        // [cap_l]      LCG state, 0 = unset
     unsigned char* flags;  // [cap_l]      bits 0-2 count, 3 selected, 4 split
     int* split_list;       // [cap_l]      leaves split by the last pass, in leaf order
+    int4* sdat;            // [cap_l/2+1][2] per split of the last pass: the children's sample
+                           //   indices (left 2, right 3) and box volumes, in split order
     int4* part;            // [kChunkBlocks] per-workgroup (selected, split, samples, carry)
     int2* part_off;        // [kChunkBlocks] exclusive prefix of (selected, split)
     uint4* epart;          // [kErrBlocks] per-workgroup (max error bits, stale, values)

@@ -101,12 +101,10 @@ struct LeafSamples {
     float vol;
 };
 
-template <int D>
-__device__ __forceinline__ float leaf_error_s(const DevGrid& a, int i, LeafSamples& ls, float* mn,
-                                              float* mx) {
-    int4 id;
-    load_leaf<D>(a, i, id, mn, mx);
-    const int ids[4] = {id.x, id.y, id.z, id.w};
+// Relative-L1 error of a leaf from its sample indices in slot order (< 0 =
+// empty) and its box volume (kdgrid.cu:238-266); fills ls.
+__device__ __forceinline__ float leaf_error_ids(const DevGrid& a, const int* ids, float vol,
+                                                LeafSamples& ls) {
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     int n = 0;
 #pragma unroll
@@ -135,9 +133,6 @@ __device__ __forceinline__ float leaf_error_s(const DevGrid& a, int i, LeafSampl
             dz += fabsf(ls.vz[q] - az);
         }
     }
-    float vol = 1.0f;
-#pragma unroll
-    for (int d = 0; d < D; ++d) vol *= (mx[d] - mn[d]);
     float e = 0.0f;
     if (ax != 0.0f) e += dx / ax;
     if (ay != 0.0f) e += dy / ay;
@@ -148,6 +143,24 @@ __device__ __forceinline__ float leaf_error_s(const DevGrid& a, int i, LeafSampl
     ls.n = n;
     ls.vol = vol;
     return e;
+}
+
+// Box volume: extents multiplied in dimension order from 1 (aabb.h:20-25).
+template <int D>
+__device__ __forceinline__ float box_volume(const float* mn, const float* mx) {
+    float vol = 1.0f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) vol *= (mx[d] - mn[d]);
+    return vol;
+}
+
+template <int D>
+__device__ __forceinline__ float leaf_error_s(const DevGrid& a, int i, LeafSamples& ls, float* mn,
+                                              float* mx) {
+    int4 id;
+    load_leaf<D>(a, i, id, mn, mx);
+    const int ids[4] = {id.x, id.y, id.z, id.w};
+    return leaf_error_ids(a, ids, box_volume<D>(mn, mx), ls);
 }
 
 // *nvals returns the number of sample values read.
