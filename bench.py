@@ -242,26 +242,33 @@ def run_frames(log=print):
     e = L.extra_bits(W, H, D, c["b"])
     target = c["spp"] * W * H
     cap = int(target * 1.25) + 4 * (1 << (c["b"] * D + 2 * e)) // 8
-    # device bytes of one frame's grid: samples (coordinates, values) and per
-    # leaf the record, the padded point-copy row, placement list entry and
-    # LCG state, error, state, flags, split list, learned features and plain
-    # error, reconstruction record and ~1.2 sort keys (twice)
-    leaf_rec = 16 * (1 + (2 * D + 3) // 4)
-    pts_row = 4 * (16 if 4 * D <= 16 else 32)
-    per_leaf = leaf_rec + pts_row + 8 + 4 + 4 + 4 + 1 + 4 + 4 * 16 + 4 + 32 + 2 * 8 * 1.2
-    per_frame = int(cap * (4 * D + 12) + (cap // 2) * per_leaf)
-    free, _ = hip.mem_info()
-    F = int(max(1, min(c["frames"], (free * 0.9) // per_frame)))
-    log(f"[bench] frames: {per_frame / 2**30:.0f} GiB per frame grid, {free / 2**30:.0f} GiB free, "
-        f"{F} frame(s) per batch")
-    grids = [kdgrid.KDGrid(D, cap, c["cand"], c["b"], e, c["alpha"], float(W), float(H))
-             for _ in range(F)]
-    for g in grids:
+    # one warm-up frame on the first grid, outside the timed region: it picks
+    # the gather tile and, with its reconstruction buffers sized, tells how
+    # much device memory a frame's grid really takes, hence how many frames
+    # a batch can hold
+    px = hip.DeviceBuffer(W * H * 16)
+    hip.synchronize()
+    free0, _ = hip.mem_info()
+
+    def new_grid():
+        g = kdgrid.KDGrid(D, cap, c["cand"], c["b"], e, c["alpha"], float(W), float(H))
         g.set_learned(2, net)
         g.set_learned_exponent(kappa)
-    px = hip.DeviceBuffer(W * H * 16)
+        return g
+
+    grids = [new_grid()]
+    sc = kdgrid.Scene(c["kind"], c["first_seed"], D, float(W), float(H))
+    grids[0].Run(sc, target)
+    tile = grids[0].autotune_recon(px.ptr, W, H)
+    sc.close()
+    hip.synchronize()
+    free1, _ = hip.mem_info()
+    per_frame = max(1, free0 - free1)
+    F = int(max(1, min(c["frames"], 1 + (free1 - (4 << 30)) // per_frame)))
+    log(f"[bench] frames: {per_frame / 2**30:.1f} GiB per frame grid, {free0 / 2**30:.0f} GiB free, "
+        f"{F} frame(s) per batch, gather tile {tile}")
+    grids += [new_grid() for _ in range(F - 1)]
     samples, wall = 0, 0.0
-    tile = None
     for f0 in range(0, c["frames"], F):
         fr = list(range(f0, min(c["frames"], f0 + F)))
         scenes = [kdgrid.Scene(c["kind"], c["first_seed"] + f, D, float(W), float(H)) for f in fr]
@@ -270,8 +277,7 @@ def run_frames(log=print):
         t1 = time.perf_counter()
         batch.run([target] * len(fr), use_graph=True)
         for g in grids[:len(fr)]:
-            if tile is None:
-                tile = g.autotune_recon(px.ptr, W, H)
+            g.set_recon_tile(*tile)
             g.Integrate(px.ptr, W, H)
             samples += g.GetNumberOfSamples()
         hip.synchronize()
