@@ -604,8 +604,10 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
 #pragma unroll
             for (int d = 0; d < D; ++d) dst[d] = pt[d];
         }
-        a.err[i] = -1.0f;
-        a.err[m] = -1.0f;
+        if (!a.in_loop) {  // in the loop the next error pass overwrites both
+            a.err[i] = -1.0f;
+            a.err[m] = -1.0f;
+        }
         a.seed[m] = 0u;
         a.flags[m] = 3;
         a.split_list[m - L] = i;  // the split leaves in leaf order (right child m)
@@ -687,6 +689,25 @@ __global__ __launch_bounds__(256) void k_place(DevGrid a) {
 template <int D>
 __global__ __launch_bounds__(256) void k_place_b(const DevGrid* grids) {
     place_body<D>(grids[blockIdx.y], blockIdx.x, gridDim.x);
+}
+
+// After the device-side loop: when it stopped at its limit, no error pass
+// followed the last placement, so its split children get the stale marker
+// (error -1, kdgrid.cu:466-469) the placement left out.
+__device__ __forceinline__ void mark_last_children_body(const DevGrid& a, int blk, int nblk) {
+    const DevCounters* c = a.cnt;
+    if (c->status != 0 || c->n_samples < c->limit) return;
+    const int nn = c->new_nodes, L_old = c->n_leaves_old;
+    for (int t = blk * 256 + threadIdx.x; t < nn; t += nblk * 256) {
+        a.err[a.split_list[t]] = -1.0f;
+        a.err[L_old + t] = -1.0f;
+    }
+}
+__global__ __launch_bounds__(256) void k_mark_last_children(DevGrid a) {
+    mark_last_children_body(a, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(256) void k_mark_last_children_b(const DevGrid* grids) {
+    mark_last_children_body(grids[blockIdx.y], blockIdx.x, gridDim.x);
 }
 
 // Per-leaf point copies from the sample coordinates (after a state upload).
@@ -787,6 +808,16 @@ hipError_t launch_adaptive(Grid& g, hipStream_t st) {
     if (e == hipSuccess) e = launch_scan(g, st);
     if (e == hipSuccess) e = launch_place(g, st);
     return e;
+}
+
+hipError_t launch_mark_last_children(Grid& g, hipStream_t st) {
+    hipLaunchKernelGGL(k_mark_last_children, dim3(1024), dim3(256), 0, st, g.dev);
+    return hipGetLastError();
+}
+
+hipError_t launch_mark_last_children_b(const DevGrid* grids, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_mark_last_children_b, dim3(256, n), dim3(256), 0, st, grids);
+    return hipGetLastError();
 }
 
 hipError_t launch_build_pts(Grid& g, int n_leaves, hipStream_t st) {

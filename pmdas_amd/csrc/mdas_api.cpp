@@ -123,7 +123,7 @@ void collect_kernel_times(Grid& g, std::vector<std::pair<std::string, double>>* 
     };
     g.t_initial = get("k_initial");
     g.t_errors = get("k_errors");
-    g.t_adaptive = get("k_select") + get("k_scan_counts") + get("k_place");
+    g.t_adaptive = get("k_select") + get("k_scan_counts") + get("k_place") + get("k_mark_last");
     g.t_integrand = get("k_eval");
 }
 
@@ -685,6 +685,11 @@ int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream)
     HIP_TRY(mdas::launch_eval(g, s->dev, 0, 4 * L0, false, st));
     mdas::kend(g, st);
     int enqueued = 0;
+    struct LoopMode {  // the pass kernels of this loop skip the stale markers
+        mdas::DevGrid& d;
+        explicit LoopMode(mdas::DevGrid& x) : d(x) { d.in_loop = 1; }
+        ~LoopMode() { d.in_loop = 0; }
+    } loop_mode{g.dev};
     for (;;) {
         for (int b = 0; b < B; ++b) {
             mdas::kbeg(g, st, "k_errors");
@@ -712,6 +717,9 @@ int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream)
         // pathological integrand could make that one sample per pass
         if (enqueued >= 20000) return fail(MDAS_ERR_STATE, "sampling loop exceeded 20000 passes");
     }
+    mdas::kbeg(g, st, "k_mark_last");
+    HIP_TRY(mdas::launch_mark_last_children(g, st));
+    mdas::kend(g, st);
     g.passes = g.host.passes;
     std::vector<std::pair<std::string, double>> seq;
     collect_kernel_times(g, logging ? &seq : nullptr);

@@ -88,6 +88,7 @@ int mdas_batch_create(mdas_handle* grids, mdas_scene_handle* scenes, int n, mdas
         bool learned = false;
         for (int k = 0; k < m; ++k) {
             dg[k] = grids[gr.members[k]]->g.dev;
+            dg[k].in_loop = 1;  // batches only run the device-side loop
             dp[k] = scenes[gr.members[k]]->dev;
             dl[k] = grids[gr.members[k]]->g.learn;  // feat == null: plain error for this member
             learned = learned || grids[gr.members[k]]->g.learn_mode != 0;
@@ -211,6 +212,8 @@ int mdas_batch_run(mdas_batch_handle b, const long long* targets, void* stream, 
         if (all) break;
         if (blocks_run * B >= 20000) return mdas::set_error(MDAS_ERR_STATE, "batch exceeded 20000 passes");
     }
+    for (Group& g : b->groups)
+        B_TRY(mdas::launch_mark_last_children_b(g.d_grids, (int)g.members.size(), st));
     for (int i = 0; i < n; ++i) {
         mdas::Grid& gg = b->grids[i]->g;
         int rc = mdas::sync_grid(gg, st);

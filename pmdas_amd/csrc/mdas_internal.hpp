@@ -84,6 +84,8 @@ struct DevGrid {
     int cap_s, cap_l;
     int cand;
     unsigned jmul, jadd, jinv;  // LCG jump over candidatesNum * D draws and its inverse
+    int in_loop;                // 1 inside the device-side loop: the placement leaves the
+                                //   children's stale markers to the next error pass
     int b, e;
     float alpha, sX, sY;
 };
@@ -197,6 +199,8 @@ hipError_t launch_place(Grid& g, hipStream_t st);
 hipError_t launch_set_limit(Grid& g, int limit, hipStream_t st);
 hipError_t launch_arm(Grid& g, hipStream_t st);
 hipError_t launch_build_pts(Grid& g, int n_leaves, hipStream_t st);
+hipError_t launch_mark_last_children(Grid& g, hipStream_t st);
+hipError_t launch_mark_last_children_b(const DevGrid* grids, int n, hipStream_t st);
 hipError_t launch_fill_i32(int* p, int v, long long n, hipStream_t st);
 hipError_t launch_fill_f32(float* p, float v, long long n, hipStream_t st);
 
