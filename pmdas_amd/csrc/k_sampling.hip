@@ -616,21 +616,38 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
     // selection and compaction kernels
 }
 
-// Compaction: the selected leaves of a workgroup's chunk (the selection's
-// partition) are ranked in leaf order by wave ballots; entry r of the
-// placement list (r = the leaf's rank among all selected leaves, so its new
-// sample is N_old + r) gets (leaf | count << 30, right child or -1); leaf
-// indices are below 2^30, and a split leaf's count is 4.  Every
-// leaf's count is rewritten here as it stands after the placement, in full
-// coalesced lines.
-__device__ __forceinline__ void compact_body(const DevGrid& a, int blk, int nblk) {
+// Compaction and placement in one kernel.  Each workgroup ranks the selected
+// leaves of its chunk (the selection's partition) in leaf order by wave
+// ballots: a leaf of global rank r (r = the rank among all selected leaves)
+// gets the new sample N_old + r, a split leaf of split rank q the right child
+// L_old + q.  The entries (leaf | count << 30, right child or -1; leaf
+// indices are below 2^30, a split leaf's count is 4) and the LCG states
+// they start from queue up in LDS and are placed a round of 256 at a time,
+// one per thread.  Every leaf's count is rewritten in the same sweep, as it
+// stands after the placement, in full coalesced lines.
+template <int D>
+__device__ __forceinline__ void compact_place_body(const DevGrid& a, int blk, int nblk) {
+    constexpr int U = 4;                   // tiles whose flags and states are loaded together
+    constexpr int QCAP = U * 256 + 256;    // one step's entries plus an unplaced remainder
     __shared__ int lds[8];
+    __shared__ int2 qe[QCAP];
+    __shared__ unsigned qs[QCAP];
     if (a.cnt->done) return;
     const int L = a.cnt->n_leaves_old;
+    const int N0 = a.cnt->n_samples_old;
     int beg, end;
     chunk_range(L, nblk, blk, beg, end);
     int2 base = a.part_off[blk];
-    constexpr int U = 4;  // tiles whose flags and states are loaded together
+    int jq = base.x;  // global rank of the entry at the front of the queue
+    int qn = 0;       // queued entries
+    auto place_round = [&](int p0, int count) {
+        const int p = p0 + (int)threadIdx.x;
+        if ((int)threadIdx.x < count) {
+            const int2 q = qe[p];
+            const unsigned cnt = q.y >= 0 ? 4u : ((unsigned)q.x >> 30);
+            place_leaf<D>(a, q.x & 0x3fffffff, N0 + jq + p, q.y, cnt, L, qs[p]);
+        }
+    };
     for (int t00 = beg; t00 < end; t00 += U * 256) {
         unsigned fv[U], sv[U];
 #pragma unroll
@@ -649,46 +666,47 @@ __device__ __forceinline__ void compact_body(const DevGrid& a, int blk, int nblk
             const unsigned cnt = f & 7u;
             int rs, rl, ts, tl;
             block_rank2(sel, split, lds, rs, rl, ts, tl);
-            if (sel) {  // the new sample is N0 + list index; a split leaf holds 4
-                a.qlist[base.x + rs] =
-                    make_int2(i | (int)((cnt & 3u) << 30), split ? L + base.y + rl : -1);
+            if (sel) {
+                qe[qn + rs] = make_int2(i | (int)((cnt & 3u) << 30), split ? L + base.y + rl : -1);
                 // the selection stored the state after the candidate draws;
                 // the placement starts from the state before them
-                a.qseed[base.x + rs] = a.jinv * (sv[r] - a.jadd);
+                qs[qn + rs] = a.jinv * (sv[r] - a.jadd);
             }
             if (i < end) a.flags[i] = (unsigned char)(split ? 2u : (sel ? cnt + 1u : cnt));
-            base.x += ts;
+            qn += ts;
             base.y += tl;
         }
+        __syncthreads();
+        int placed = 0;
+        for (; qn - placed >= 256; placed += 256) place_round(placed, 256);
+        if (placed > 0) {  // the remainder (< 256) to the front of the queue
+            const int rem = qn - placed;
+            int2 e = make_int2(0, 0);
+            unsigned st = 0u;
+            if ((int)threadIdx.x < rem) {
+                e = qe[placed + threadIdx.x];
+                st = qs[placed + threadIdx.x];
+            }
+            __syncthreads();
+            if ((int)threadIdx.x < rem) {
+                qe[threadIdx.x] = e;
+                qs[threadIdx.x] = st;
+            }
+            jq += placed;
+            qn = rem;
+        }
+        __syncthreads();
     }
-}
-
-__global__ __launch_bounds__(256) void k_compact(DevGrid a) { compact_body(a, blockIdx.x, gridDim.x); }
-__global__ __launch_bounds__(256) void k_compact_b(const DevGrid* grids) {
-    compact_body(grids[blockIdx.y], blockIdx.x, gridDim.x);
-}
-
-// Placement: one thread per list entry, every lane busy.
-template <int D>
-__device__ __forceinline__ void place_body(const DevGrid& a, int blk, int nblk) {
-    if (a.cnt->done) return;
-    const int L = a.cnt->n_leaves_old;
-    const int N0 = a.cnt->n_samples_old;
-    const int n = a.cnt->new_samples;
-    for (int j = blk * 256 + threadIdx.x; j < n; j += nblk * 256) {
-        const int2 q = a.qlist[j];
-        const unsigned cnt = q.y >= 0 ? 4u : ((unsigned)q.x >> 30);
-        place_leaf<D>(a, q.x & 0x3fffffff, N0 + j, q.y, cnt, L, a.qseed[j]);
-    }
+    place_round(0, qn);
 }
 
 template <int D>
-__global__ __launch_bounds__(256) void k_place(DevGrid a) {
-    place_body<D>(a, blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(256) void k_compact_place(DevGrid a) {
+    compact_place_body<D>(a, blockIdx.x, gridDim.x);
 }
 template <int D>
-__global__ __launch_bounds__(256) void k_place_b(const DevGrid* grids) {
-    place_body<D>(grids[blockIdx.y], blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(256) void k_compact_place_b(const DevGrid* grids) {
+    compact_place_body<D>(grids[blockIdx.y], blockIdx.x, gridDim.x);
 }
 
 // After the device-side loop: when it stopped at its limit, no error pass
@@ -798,8 +816,8 @@ hipError_t launch_scan(Grid& g, hipStream_t st) {
 }
 
 hipError_t launch_place(Grid& g, hipStream_t st) {
-    hipLaunchKernelGGL(k_compact, dim3(g.chunk_blocks), dim3(256), 0, st, g.dev);
-    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_place<DD>, dim3(kPlaceBlocks), dim3(256), 0, st, g.dev));
+    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_compact_place<DD>, dim3(g.chunk_blocks), dim3(256), 0, st,
+                                            g.dev));
     return hipGetLastError();
 }
 
@@ -859,8 +877,7 @@ hipError_t launch_pass_b(int D, const DevGrid* grids, int n, int blocks, hipStre
     }
     hipLaunchKernelGGL(k_select_b, dim3(blocks, n), dim3(256), 0, st, grids);
     hipLaunchKernelGGL(k_scan_counts_b, dim3(n), dim3(1024), 0, st, grids, blocks);
-    hipLaunchKernelGGL(k_compact_b, dim3(blocks, n), dim3(256), 0, st, grids);
-    MDAS_DISPATCH_D(D, hipLaunchKernelGGL(k_place_b<DD>, dim3(blocks, n), dim3(256), 0, st, grids));
+    MDAS_DISPATCH_D(D, hipLaunchKernelGGL(k_compact_place_b<DD>, dim3(blocks, n), dim3(256), 0, st, grids));
     return hipGetLastError();
 }
 

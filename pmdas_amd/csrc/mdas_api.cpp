@@ -58,7 +58,7 @@ void lcg_jump(unsigned k, unsigned& A, unsigned& C, unsigned& Ainv) {
 
 void free_grid(Grid& g) {
     mdas::DevGrid& d = g.dev;
-    void* ptrs[] = {d.coords,   d.values, d.leaf,     d.err,  d.seed,       d.flags, d.pts, d.qlist, d.qseed,
+    void* ptrs[] = {d.coords,   d.values, d.leaf,     d.err,  d.seed,       d.flags, d.pts,
                     d.split_list, d.sdat, d.part, d.part_off, d.epart, d.rrec,      d.cnt,
                     d.pstats,   g.keys,   g.keys_alt, g.tile_start, g.tile_end, g.rs_hist,
                     g.d_scalar, g.learn.feat, g.learn.perr, g.learn.rows};
@@ -182,8 +182,6 @@ int mdas_create(const mdas_config* cfg, mdas_handle* out) {
     if (e == hipSuccess) e = dalloc(&d.values, cs * 3);
     if (e == hipSuccess) e = dalloc(&d.leaf, cl * mdas::leaf_quads(D));
     if (e == hipSuccess) e = dalloc(&d.pts, cl * mdas::pts_stride(D));
-    if (e == hipSuccess) e = dalloc(&d.qlist, cl);
-    if (e == hipSuccess) e = dalloc(&d.qseed, cl);
     if (e == hipSuccess) e = dalloc(&d.err, cl);
     if (e == hipSuccess) e = dalloc(&d.seed, cl);
     if (e == hipSuccess) e = dalloc(&d.flags, cl);
@@ -789,11 +787,10 @@ static double kernel_bytes(const Grid& g, const std::string& k) {
             b += 14.0 * p.L;
         else if (k == "k_scan_counts")
             b += (16.0 + 8.0) * g.chunk_blocks;
-        else if (k == "k_place")  // compaction + list placement
-            b += 2.0 * p.L +                                              // flags read, count written
-                 (double)p.sel * (12.0 + 12.0 + 4.0 + LB + 4.0 * D) +     // list (entry + start state)
-                                                                          // written + read, state,
-                                                                          // record, new sample
+        else if (k == "k_place")  // compaction and placement
+            b += 6.0 * p.L +                                              // flags, states read; counts
+                                                                          // written
+                 (double)p.sel * (LB + 4.0 * D) +                         // record, new sample
                  4.0 * D * p.selcnt +                                     // the leaf's points
                  (double)(p.sel - p.split) * (16.0 + 4.0 * D) +           // slots + point of an insert
                  (double)p.split * (2.0 * LB + 20.0 * D + 8.0 + 4.0 + 1.0 + 4.0 + 32.0);  // two

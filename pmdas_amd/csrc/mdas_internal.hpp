@@ -45,7 +45,6 @@ struct KRec {
 
 static const int kChunkBlocks = 32768;  // max workgroups of the chunked leaf kernels
 static const int kErrBlocks = 8192;     // max workgroups of the error kernels
-static const int kPlaceBlocks = 4096;   // workgroups of the list placement
 static const int kMaxPassStats = 4096;
 
 // Float4 quads of one leaf record: the int4 of sample slots, then the box
@@ -65,8 +64,6 @@ struct DevGrid {
     float* values;         // [cap_s * 3]  float3 AoS
     float4* leaf;          // [cap_l * leaf_quads(D)]  slots (~D = empty) + box
     float* pts;            // [cap_l * pts_stride(D)]  copy of each leaf's sample points by slot
-    int2* qlist;           // [cap_l]      placement list (leaf | count << 30, right child)
-    unsigned* qseed;       // [cap_l]      placement list: LCG state before the leaf's candidate draws
     float* err;            // [cap_l]      < 0 = stale
     unsigned* seed;        // [cap_l]      LCG state, 0 = unset
     unsigned char* flags;  // [cap_l]      bits 0-2 count, 3 selected, 4 split
