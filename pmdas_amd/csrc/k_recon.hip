@@ -81,15 +81,23 @@ __global__ __launch_bounds__(256) void k_recon_prep(DevGrid a, ReconGeo G) {
         float mn[D], mx[D];
         load_leaf<D>(a, i, id4, mn, mx);
         const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
+        // all four loads in flight at once; an empty slot reads sample 0
+        float vx[4], vy[4], vz[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float* v = a.values + (size_t)(ids[q] >= 0 ? ids[q] : 0) * 3;
+            vx[q] = v[0];
+            vy[q] = v[1];
+            vz[q] = v[2];
+        }
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         int n = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             if (ids[q] >= 0) {
-                const float* v = a.values + (size_t)ids[q] * 3;
-                sx += v[0];
-                sy += v[1];
-                sz += v[2];
+                sx += vx[q];
+                sy += vy[q];
+                sz += vz[q];
                 ++n;
             }
         const float inv = 1.0f / (float)n;

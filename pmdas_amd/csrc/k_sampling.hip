@@ -201,16 +201,18 @@ __device__ __forceinline__ void errors_stale_body(const DevGrid& a, int blk, int
         const int L_old = a.cnt->n_leaves_old;
         for (int t = blk * 256 + threadIdx.x; t < nn; t += nblk * 256) {
             const int4 s0 = a.sdat[2 * t], s1 = a.sdat[2 * t + 1];
-            LeafSamples ls;
+            const int il = a.split_list[t];
+            LeafSamples lsl, lsr;
             const int idl[4] = {s0.x, s0.y, -1, -1};
-            const float el = leaf_error_ids(a, idl, __int_as_float(s1.y), ls);
-            const int nl = ls.n;
             const int idr[4] = {s0.z, s0.w, s1.x, -1};
-            const float er = leaf_error_ids(a, idr, __int_as_float(s1.z), ls);
-            a.err[a.split_list[t]] = el;
+            load_values(a, idl, lsl);  // the five value loads together
+            load_values(a, idr, lsr);
+            const float el = error_of_values(idl, __int_as_float(s1.y), lsl);
+            const float er = error_of_values(idr, __int_as_float(s1.z), lsr);
+            a.err[il] = el;
             a.err[L_old + t] = er;
             n_stale += 2;
-            n_vals += nl + ls.n;
+            n_vals += lsl.n + lsr.n;
             m = fmaxf(m, el);
             m = fmaxf(m, er);
         }

@@ -101,20 +101,30 @@ struct LeafSamples {
     float vol;
 };
 
-// Relative-L1 error of a leaf from its sample indices in slot order (< 0 =
-// empty) and its box volume (kdgrid.cu:238-266); fills ls.
-__device__ __forceinline__ float leaf_error_ids(const DevGrid& a, const int* ids, float vol,
-                                                LeafSamples& ls) {
+// The values of a leaf's samples by slot (indices < 0 = empty).  The four
+// loads go out together: an empty slot reads sample 0 (always present) and
+// is zeroed after, so no load waits behind a branch.
+__device__ __forceinline__ void load_values(const DevGrid& a, const int* ids, LeafSamples& ls) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float* v = a.values + (size_t)(ids[q] >= 0 ? ids[q] : 0) * 3;
+        ls.vx[q] = v[0];
+        ls.vy[q] = v[1];
+        ls.vz[q] = v[2];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (ids[q] < 0) ls.vx[q] = ls.vy[q] = ls.vz[q] = 0.0f;
+}
+
+// Relative-L1 error of a leaf from its loaded slot values (the occupied slots
+// come first) and its box volume (kdgrid.cu:238-266); sets ls.n and ls.vol.
+__device__ __forceinline__ float error_of_values(const int* ids, float vol, LeafSamples& ls) {
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     int n = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        ls.vx[q] = ls.vy[q] = ls.vz[q] = 0.0f;
         if (ids[q] >= 0) {
-            const float* v = a.values + (size_t)ids[q] * 3;
-            ls.vx[q] = v[0];
-            ls.vy[q] = v[1];
-            ls.vz[q] = v[2];
             sx += ls.vx[q];
             sy += ls.vy[q];
             sz += ls.vz[q];
@@ -143,6 +153,13 @@ __device__ __forceinline__ float leaf_error_ids(const DevGrid& a, const int* ids
     ls.n = n;
     ls.vol = vol;
     return e;
+}
+
+// Both in one: the error of a leaf from its sample indices and volume.
+__device__ __forceinline__ float leaf_error_ids(const DevGrid& a, const int* ids, float vol,
+                                                LeafSamples& ls) {
+    load_values(a, ids, ls);
+    return error_of_values(ids, vol, ls);
 }
 
 // Box volume: extents multiplied in dimension order from 1 (aabb.h:20-25).
