@@ -171,11 +171,12 @@ __global__ __launch_bounds__(256) void k_errors(DevGrid a) {
 }
 
 // The same result inside the sampling loop without the sweep over all
-// leaves: the only leaves with error < 0 are the two children of every leaf
-// the last pass split (split_list[t] and L_old + t, whose sample indices and
-// volumes the placement left in the split's entry); the max over the other
-// leaves was carried by the selection kernel into gmax by the allocation
-// scan.  The first pass after the initial grid re-estimates every leaf.
+// leaves: the only stale leaves are the two children of every leaf the last
+// pass split (split_list[t] and L_old + t, whose sample indices and volumes
+// the placement left in the split's entry; inside the loop it does not mark
+// them -1, since they are re-estimated here); the max over the other leaves
+// was carried by the selection kernel into gmax by the allocation scan.  The
+// first pass after the initial grid re-estimates every leaf.
 template <int D>
 __device__ __forceinline__ void errors_stale_body(const DevGrid& a, int blk, int nblk) {
     __shared__ float wmax[4];
