@@ -6,7 +6,12 @@ reconstruction and its sRGB bytes against the oracle, bit for bit.
   spp, so no adaptive pass runs; and the variant bitsPerDim 0 (4 initial spp)
   that does run passes;
 - 1920x1080 4D depth of field, 16 spp (the 4D benchmark line);
-- 1920x1080 5D motion blur + depth of field, 32 spp (the headline workload).
+- 1920x1080 5D motion blur + depth of field, 32 spp (the headline workload);
+- the 32-image batch of uneven sizes (746 M samples) against the same images
+  run one at a time;
+- one 3840x2160 6D ambient-occlusion frame at 64 spp with learned scoring
+  (609 M samples, indices past 2^31): tree invariants, integrand values across
+  the index range, the image's integral.
 """
 import sys
 
