@@ -259,6 +259,11 @@ typedef struct mdas_mlp* mdas_mlp_handle;
  * sum over children c of (m_c - m_p)^2 v_c^2 / a_c (m mean luminance, v box
  * volume, a xy footprint area); scored as 2^clamp(prediction, -120, 120). */
 #define MDAS_LEARN_IMAGE_LABEL 4
+/* Keep every leaf's full feature row and plain error for
+ * mdas_get_leaf_features (always kept while collecting); without it a
+ * scoring run keeps only each leaf's depth, the one feature a split carries
+ * over to its children. */
+#define MDAS_LEARN_KEEP_FEATURES 8
 
 /* Seeded default initialisation: U(-1/sqrt(fan_in), 1/sqrt(fan_in)). */
 int mdas_mlp_create(uint32_t seed, mdas_mlp_handle* out);
@@ -299,7 +304,8 @@ int mdas_set_learned_exponent(mdas_handle h, float kappa);
 int mdas_set_learned_exponents(mdas_handle h, float kappa_e, float kappa_y);
 /* Rows offered since mdas_set_learned (those beyond cap_rows were dropped). */
 int mdas_get_learned_rows(mdas_handle h, long long* offered);
-/* Features and plain errors of leaves [0, n) after a learned-mode run. */
+/* Features and plain errors of leaves [0, n) after a learned-mode run that
+ * kept them (MDAS_LEARN_COLLECT or MDAS_LEARN_KEEP_FEATURES). */
 int mdas_get_leaf_features(mdas_handle h, float* feat_host, float* perr_host, long long n);
 
 const char* mdas_last_error(void);

@@ -29,6 +29,7 @@ FEATURES = 16
 LEARN_COLLECT = 1
 LEARN_SCORE = 2
 LEARN_IMAGE_LABEL = 4
+LEARN_KEEP_FEATURES = 8
 
 
 class MdasError(RuntimeError):

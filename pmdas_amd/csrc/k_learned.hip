@@ -128,10 +128,16 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
         float depth = 0.0f, pe = 0.0f;
         float pf[kFeat];
         if (act && !first && feats) {
+            if (l.keep) {  // the parent's row (still in the left child's slot)
 #pragma unroll
-            for (int k = 0; k < kFeat; ++k) pf[k] = l.feat[(size_t)c[0] * kFeat + k];
+                for (int k = 0; k < kFeat; ++k) pf[k] = l.feat[(size_t)c[0] * kFeat + k];
+                pe = l.perr[c[0]];
+            } else {  // scoring only: the parent's depth is all a split carries over
+#pragma unroll
+                for (int k = 0; k < kFeat; ++k) pf[k] = 0.0f;
+                pf[11] = l.feat[(size_t)c[0] * kFeat + 11];
+            }
             depth = pf[11] + 1.0f;
-            pe = l.perr[c[0]];
         }
         float e[2] = {0.0f, 0.0f}, vol[2] = {1.0f, 1.0f}, axy[2] = {1.0f, 1.0f};
         float f[2][kFeat];
@@ -167,11 +173,15 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             if (c[h] >= 0 && feats) {
-                float4* fo = (float4*)(l.feat + (size_t)c[h] * kFeat);
+                if (l.keep) {
+                    float4* fo = (float4*)(l.feat + (size_t)c[h] * kFeat);
 #pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    fo[k] = make_float4(f[h][4 * k], f[h][4 * k + 1], f[h][4 * k + 2], f[h][4 * k + 3]);
-                l.perr[c[h]] = e[h];
+                    for (int k = 0; k < 4; ++k)
+                        fo[k] = make_float4(f[h][4 * k], f[h][4 * k + 1], f[h][4 * k + 2], f[h][4 * k + 3]);
+                    l.perr[c[h]] = e[h];
+                } else {
+                    l.feat[(size_t)c[h] * kFeat + 11] = f[h][11];
+                }
             }
         }
         // scores: the wave's (up to) 128 leaves as 8 tiles of 16 rows

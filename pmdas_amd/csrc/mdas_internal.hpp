@@ -113,6 +113,7 @@ struct LearnDev {
     float ylo, yhi;      // clamp of the network output
     int label_mode;      // 0: relative error reduction, score e 2^y; 1: image change, score 2^y
     float kappa_e, kappa_y;  // score = e^kappa_e * 2^(kappa_y * y)
+    int keep;            // 1: full feature rows and plain errors; 0: only each leaf's depth
 };
 
 struct Grid {

@@ -256,7 +256,7 @@ int mdas_mlp_train(mdas_mlp_handle m, const float* rows, const float* targets, l
 int mdas_set_learned(mdas_handle h, int mode, mdas_mlp_handle m, float* rows, float* labels,
                      long long cap_rows) {
     if (!h) return set_error(MDAS_ERR_INVALID, "null handle");
-    if (mode & ~(MDAS_LEARN_COLLECT | MDAS_LEARN_SCORE | MDAS_LEARN_IMAGE_LABEL))
+    if (mode & ~(MDAS_LEARN_COLLECT | MDAS_LEARN_SCORE | MDAS_LEARN_IMAGE_LABEL | MDAS_LEARN_KEEP_FEATURES))
         return set_error(MDAS_ERR_INVALID, "unknown learned mode bits");
     if ((mode & MDAS_LEARN_SCORE) && !m)
         return set_error(MDAS_ERR_INVALID, "scoring needs a network");
@@ -275,6 +275,7 @@ int mdas_set_learned(mdas_handle h, int mode, mdas_mlp_handle m, float* rows, fl
     g.learn.cap_rows = (mode & MDAS_LEARN_COLLECT) ? cap_rows : 0;
     g.learn.mlp = (mode & MDAS_LEARN_SCORE) ? m->dev() : mdas::MlpDev{};
     g.learn.label_mode = (mode & MDAS_LEARN_IMAGE_LABEL) ? 1 : 0;
+    g.learn.keep = (mode & (MDAS_LEARN_COLLECT | MDAS_LEARN_KEEP_FEATURES)) ? 1 : 0;
     g.learn.kappa_e = g.learn.label_mode ? 0.0f : 1.0f;
     g.learn.kappa_y = 1.0f;
     g.learn.ylo = g.learn.label_mode ? -120.0f : -10.0f;
@@ -314,6 +315,8 @@ int mdas_get_leaf_features(mdas_handle h, float* feat_host, float* perr_host, lo
     if (!h) return set_error(MDAS_ERR_INVALID, "null handle");
     const mdas::Grid& g = h->g;
     if (!g.learn.feat) return set_error(MDAS_ERR_STATE, "learned mode was never enabled");
+    if (!g.learn.keep)
+        return set_error(MDAS_ERR_STATE, "features are kept with MDAS_LEARN_COLLECT or MDAS_LEARN_KEEP_FEATURES");
     if (n < 0 || n > g.cap_l) return set_error(MDAS_ERR_INVALID, "leaf count out of range");
     HIP_OK(hipDeviceSynchronize());
     if (feat_host)

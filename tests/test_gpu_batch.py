@@ -97,7 +97,7 @@ def test_batch_learned_equals_single_runs(kd):
     made = [_make(kd, s) for s in specs]
     for (g, *_), on in zip(made, learned):
         if on:
-            g.set_learned(2, net)
+            g.set_learned(2 | 8, net)
             g.set_learned_exponent(0.5)
     batch = kd.Batch([m[0] for m in made], [m[1] for m in made])
     targets = [m[2] for m in made]
@@ -108,7 +108,7 @@ def test_batch_learned_equals_single_runs(kd):
             st = g.download_state()
             solo = kd.KDGrid(*args)
             if on:
-                solo.set_learned(2, net)
+                solo.set_learned(2 | 8, net)
                 solo.set_learned_exponent(0.5)
             solo.Run(sc, target)
             ref = solo.download_state()

@@ -151,7 +151,7 @@ def test_score_of_constant_network_equals_plain(mdas):
     m = mdas.Mlp(0)
     m.set_params(p)
     g1, sc1, _, _ = _grid(mdas, CASE)
-    g1.set_learned(2, m)
+    g1.set_learned(2 | 8, m)  # score, keep the features
     g1.Run(sc1, target)
     s1 = g1.download_state()
     for k in ("coords", "values", "idx", "box", "seed"):
@@ -224,7 +224,7 @@ def test_learned_scores_match_oracle(mdas):
     m = mdas.Mlp(0)
     m.set_params(p)
     g, sc, target, _ = _grid(mdas, CASE)
-    g.set_learned(2, m)
+    g.set_learned(2 | 8, m)  # score, keep the features
     g.Run(sc, target)
     assert g.GetNumberOfSamples() >= target
     assert g.Validate()
@@ -235,6 +235,29 @@ def test_learned_scores_match_oracle(mdas):
     want = pe[live].astype(np.float64) * np.exp2(y)
     rel = np.abs(st["err"][live] - want) / want
     assert np.median(rel) < 1e-4 and rel.max() < 5e-2, (np.median(rel), rel.max())
+    m.close()
+
+
+def test_keeping_features_does_not_change_the_run(mdas):
+    """A scoring run that keeps only each leaf's depth equals one that keeps
+    the full feature rows, bit for bit; reading features back needs them kept."""
+    _say("depth-only scoring")
+    p = _params(6, scale=0.5)
+    m = mdas.Mlp(0)
+    m.set_params(p)
+    states = []
+    for mode in (2, 2 | 8):
+        g, sc, target, _ = _grid(mdas, CASE)
+        g.set_learned(mode, m)
+        g.Run(sc, target)
+        states.append(g.download_state())
+        if mode == 2:
+            with pytest.raises(Exception):
+                g.leaf_features()
+        g.close()
+        sc.close()
+    for k in states[0]:
+        assert np.array_equal(states[0][k], states[1][k]), k
     m.close()
 
 
