@@ -26,14 +26,13 @@ namespace mdas {
 __device__ __forceinline__ float log2_floor(float x) { return __log2f(fmaxf(x, 1.0e-30f)); }
 
 template <int D>
-__device__ __forceinline__ void leaf_features(const LeafSamples& ls, const float* mn,
-                                              const float* mx, float e, float depth, float sX,
-                                              float sY, float* f) {
+__device__ __forceinline__ void leaf_features_ext(const LeafSamples& ls, const float* ext, float e,
+                                                  float depth, float sX, float sY, float* f) {
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
         if (d < D) {
             const float dom = d == 0 ? sX : (d == 1 ? sY : 1.0f);
-            f[d] = log2_floor((mx[d] - mn[d]) / dom);
+            f[d] = log2_floor(ext[d] / dom);
         } else {
             f[d] = 0.0f;
         }
@@ -65,6 +64,17 @@ __device__ __forceinline__ void leaf_features(const LeafSamples& ls, const float
     f[13] = log2_floor(e / ls.vol);
     f[14] = log2_floor(ls.vol);
     f[15] = (float)D;
+}
+
+// The same from the leaf's box (extents mx - mn).
+template <int D>
+__device__ __forceinline__ void leaf_features(const LeafSamples& ls, const float* mn,
+                                              const float* mx, float e, float depth, float sX,
+                                              float sY, float* f) {
+    float ext[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) ext[d] = d < D ? mx[d] - mn[d] : 1.0f;
+    leaf_features_ext<D>(ls, ext, e, depth, sX, sY, f);
 }
 
 __device__ __forceinline__ float split_label(float ep, float el, float er) {
@@ -141,20 +151,46 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
         }
         float e[2] = {0.0f, 0.0f}, vol[2] = {1.0f, 1.0f}, axy[2] = {1.0f, 1.0f};
         float f[2][kFeat];
+        if (act && !first && a.sext != nullptr) {
+            // both children from the split's entries: sample indices, volumes
+            // and extents (the parent's, the axis's two halves), not records
+            // (a plain member of a batch has no extents and reads records)
+            const int4 s0 = a.sdat[2 * t], s1 = a.sdat[2 * t + 1];
+            const int4 x0 = a.sext[2 * t], x1 = a.sext[2 * t + 1];
+            const int ids[2][4] = {{s0.x, s0.y, -1, -1}, {s0.z, s0.w, s1.x, -1}};
+            LeafSamples ls[2];
+            load_values(a, ids[0], ls[0]);  // the five value loads together
+            load_values(a, ids[1], ls[1]);
+            vol[0] = __int_as_float(s1.y);
+            vol[1] = __int_as_float(s1.z);
+            float ex[2][6] = {{__int_as_float(x0.x), __int_as_float(x0.y), __int_as_float(x0.z),
+                               __int_as_float(x0.w), __int_as_float(x1.x), __int_as_float(x1.y)}};
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            if (c[h] >= 0) {
-                LeafSamples ls;
-                float mn[D], mx[D];
-                e[h] = leaf_error_s<D>(a, c[h], ls, mn, mx);
-                leaf_features<D>(ls, mn, mx, e[h], depth, a.sX, a.sY, f[h]);
-                vol[h] = ls.vol;
-                axy[h] = (mx[0] - mn[0]) * (mx[1] - mn[1]);
+            for (int d = 0; d < 6; ++d) ex[1][d] = d == x1.w ? __int_as_float(x1.z) : ex[0][d];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                e[h] = error_of_values(ids[h], vol[h], ls[h]);
+                leaf_features_ext<D>(ls[h], ex[h], e[h], depth, a.sX, a.sY, f[h]);
+                axy[h] = ex[h][0] * ex[h][1];
                 n_stale += 1;
-                n_vals += ls.n;
-            } else {
+                n_vals += ls[h].n;
+            }
+        } else {
 #pragma unroll
-                for (int k = 0; k < kFeat; ++k) f[h][k] = 0.0f;
+            for (int h = 0; h < 2; ++h) {
+                if (c[h] >= 0) {
+                    LeafSamples ls;
+                    float mn[D], mx[D];
+                    e[h] = leaf_error_s<D>(a, c[h], ls, mn, mx);
+                    leaf_features<D>(ls, mn, mx, e[h], depth, a.sX, a.sY, f[h]);
+                    vol[h] = ls.vol;
+                    axy[h] = (mx[0] - mn[0]) * (mx[1] - mn[1]);
+                    n_stale += 1;
+                    n_vals += ls.n;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < kFeat; ++k) f[h][k] = 0.0f;
+                }
             }
         }
         if (act && !first && feats && l.X != nullptr) {

@@ -590,6 +590,19 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
         a.sdat[2 * (m - L)] = make_int4(v[0], v[1], v[2], v[3]);
         a.sdat[2 * (m - L) + 1] = make_int4(v[4], __float_as_int(box_volume<D>(mn, cmx)),
                                             __float_as_int(box_volume<D>(cmn, mx)), 0);
+        if (a.sext) {  // learned mode: the children's extents for their features
+            float ex[6];
+#pragma unroll
+            for (int d = 0; d < 6; ++d) ex[d] = d < D ? (d == axis ? cmx[d] - mn[d] : diag[d]) : 0.0f;
+            a.sext[2 * (m - L)] = make_int4(__float_as_int(ex[0]), __float_as_int(ex[1]),
+                                            __float_as_int(ex[2]), __float_as_int(ex[3]));
+            float right = 0.0f;
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+                if (d == axis) right = mx[d] - cmn[d];
+            a.sext[2 * (m - L) + 1] = make_int4(__float_as_int(ex[4]), __float_as_int(ex[5]),
+                                                __float_as_int(right), axis);
+        }
         // point copies: sorted positions 0, 1 -> left child, 2..4 -> right
         float* rp = a.pts + (size_t)m * pts_stride(D);
 #pragma unroll

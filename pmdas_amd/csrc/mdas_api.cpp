@@ -59,7 +59,7 @@ void lcg_jump(unsigned k, unsigned& A, unsigned& C, unsigned& Ainv) {
 void free_grid(Grid& g) {
     mdas::DevGrid& d = g.dev;
     void* ptrs[] = {d.coords,   d.values, d.leaf,     d.err,  d.seed,       d.flags, d.pts,
-                    d.split_list, d.sdat, d.part, d.part_off, d.epart, d.rrec,      d.cnt,
+                    d.split_list, d.sdat, d.sext, d.part, d.part_off, d.epart, d.rrec,      d.cnt,
                     d.pstats,   g.keys,   g.keys_alt, g.tile_start, g.tile_end, g.rs_hist,
                     g.d_scalar, g.learn.feat, g.learn.perr, g.learn.rows};
     for (void* p : ptrs)

@@ -70,6 +70,9 @@ struct DevGrid {
     int* split_list;       // [cap_l]      leaves split by the last pass, in leaf order
     int4* sdat;            // [cap_l/2+1][2] per split of the last pass: the children's sample
                            //   indices (left 2, right 3) and box volumes, in split order
+    int4* sext;            // [cap_l/2+1][2] learned mode only (else null): per split the
+                           //   parent's box extents with the left child's at the split axis,
+                           //   the right child's extent at the axis, the axis
     int4* part;            // [kChunkBlocks] per-workgroup (selected, split, samples, carry)
     int2* part_off;        // [kChunkBlocks] exclusive prefix of (selected, split)
     uint4* epart;          // [kErrBlocks] per-workgroup (max error bits, stale, values)

@@ -269,6 +269,8 @@ int mdas_set_learned(mdas_handle h, int mode, mdas_mlp_handle m, float* rows, fl
         HIP_OK(hipMalloc((void**)&g.learn.perr, sizeof(float) * (size_t)g.cap_l));
         HIP_OK(hipMalloc((void**)&g.learn.rows, sizeof(long long)));
     }
+    if (mode != 0 && g.dev.sext == nullptr)
+        HIP_OK(hipMalloc((void**)&g.dev.sext, sizeof(int4) * 2 * ((size_t)g.cap_l / 2 + 1)));
     g.learn_mode = mode;
     g.learn.X = (mode & MDAS_LEARN_COLLECT) ? rows : nullptr;
     g.learn.Y = (mode & MDAS_LEARN_COLLECT) ? labels : nullptr;
