@@ -114,9 +114,10 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
                                                     int nblk) {
     __shared__ float wmax[4];
     __shared__ int wst[8];
-    __shared__ float xs[4][16 * kMlpIn];
-    __shared__ float acts[4][16 * kMlpH];
+    __shared__ unsigned short xsb[4][128 * kMlpIn];  // a wave's staged rows (bf16)
+    __shared__ unsigned short actb[4][16 * kMlpH];   // layer-1 activations (bf16)
     __shared__ float ys[4][16];
+    __shared__ __align__(16) unsigned short w1s[64 * 32], w2s[64 * 64];  // the bf16 weights
     if (pass_stopped(a.cnt)) return;
     const bool first = a.cnt->passes == 0;
     const int n_items = first ? a.cnt->n_leaves : a.cnt->new_nodes;
@@ -125,6 +126,13 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const bool feats = l.feat != nullptr;  // false: a plain member of a batch
     const bool model = feats && l.mlp.p != nullptr;
+    if (model) {  // the weights once per workgroup, for every wave's tiles
+        for (int k = threadIdx.x; k < 64 * 32 / 8; k += 256)
+            ((u16x8*)w1s)[k] = ((const u16x8*)l.mlp.w1p)[k];
+        for (int k = threadIdx.x; k < 64 * 64 / 8; k += 256)
+            ((u16x8*)w2s)[k] = ((const u16x8*)l.mlp.w2p)[k];
+        __syncthreads();
+    }
     float m = 0.0f;
     int n_stale = 0, n_vals = 0;
     for (int t0 = blk * 256; t0 < n_items; t0 += nblk * 256) {
@@ -135,22 +143,48 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
             c[0] = first ? t : a.split_list[t];
             c[1] = first ? -1 : L_old + t;
         }
-        float depth = 0.0f, pe = 0.0f;
-        float pf[kFeat];
+        // the parent's row is still in the left child's slot; a collected
+        // row takes it now, the label later needs only its error and mean
+        float depth = 0.0f, pe = 0.0f, pm = 0.0f;
         if (act && !first && feats) {
-            if (l.keep) {  // the parent's row (still in the left child's slot)
-#pragma unroll
-                for (int k = 0; k < kFeat; ++k) pf[k] = l.feat[(size_t)c[0] * kFeat + k];
+            const float* prow = l.feat + (size_t)c[0] * kFeat;
+            if (l.keep) {
                 pe = l.perr[c[0]];
-            } else {  // scoring only: the parent's depth is all a split carries over
+                pm = prow[7];
+                if (l.X != nullptr && rows_base + t < l.cap_rows) {
+                    const float4* pr = (const float4*)prow;
+                    float4* xo = (float4*)(l.X + (rows_base + t) * kFeat);
 #pragma unroll
-                for (int k = 0; k < kFeat; ++k) pf[k] = 0.0f;
-                pf[11] = l.feat[(size_t)c[0] * kFeat + 11];
+                    for (int k = 0; k < 4; ++k) xo[k] = pr[k];
+                }
             }
-            depth = pf[11] + 1.0f;
+            depth = prow[11] + 1.0f;  // scoring only: all a split carries over
         }
-        float e[2] = {0.0f, 0.0f}, vol[2] = {1.0f, 1.0f}, axy[2] = {1.0f, 1.0f};
-        float f[2][kFeat];
+        float e[2] = {0.0f, 0.0f}, vol[2] = {1.0f, 1.0f}, axy[2] = {1.0f, 1.0f}, mc[2] = {0.0f, 0.0f};
+        const int hmax = first ? 1 : 2;
+        // one child at a time: its features are stored and staged as the
+        // network's input row (row 64 h + lane) as soon as they exist, so a
+        // single feature row is live
+        auto finish = [&](int h, const float* f) {
+            mc[h] = f[7];
+            if (c[h] >= 0 && feats) {
+                if (l.keep) {
+                    float4* fo = (float4*)(l.feat + (size_t)c[h] * kFeat);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        fo[k] = make_float4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);
+                    l.perr[c[h]] = e[h];
+                } else {
+                    l.feat[(size_t)c[h] * kFeat + 11] = f[11];
+                }
+            }
+            if (model && h < hmax) {
+                unsigned short* xr = &xsb[w][(64 * h + lane) * kMlpIn];
+#pragma unroll
+                for (int k = 0; k < kMlpIn; ++k)
+                    xr[k] = f2bf((f[k] - l.mlp.p[kOffMu + k]) * l.mlp.p[kOffIsd + k]);
+            }
+        };
         if (act && !first && a.sext != nullptr) {
             // both children from the split's entries: sample indices, volumes
             // and extents (the parent's, the axis's two halves), not records
@@ -163,94 +197,75 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
             load_values(a, ids[1], ls[1]);
             vol[0] = __int_as_float(s1.y);
             vol[1] = __int_as_float(s1.z);
-            float ex[2][6] = {{__int_as_float(x0.x), __int_as_float(x0.y), __int_as_float(x0.z),
-                               __int_as_float(x0.w), __int_as_float(x1.x), __int_as_float(x1.y)}};
-#pragma unroll
-            for (int d = 0; d < 6; ++d) ex[1][d] = d == x1.w ? __int_as_float(x1.z) : ex[0][d];
+            const float exl[6] = {__int_as_float(x0.x), __int_as_float(x0.y), __int_as_float(x0.z),
+                                  __int_as_float(x0.w), __int_as_float(x1.x), __int_as_float(x1.y)};
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
+                float ex[6];
+#pragma unroll
+                for (int d = 0; d < 6; ++d) ex[d] = (h == 1 && d == x1.w) ? __int_as_float(x1.z) : exl[d];
+                float f[kFeat];
                 e[h] = error_of_values(ids[h], vol[h], ls[h]);
-                leaf_features_ext<D>(ls[h], ex[h], e[h], depth, a.sX, a.sY, f[h]);
-                axy[h] = ex[h][0] * ex[h][1];
+                leaf_features_ext<D>(ls[h], ex, e[h], depth, a.sX, a.sY, f);
+                axy[h] = ex[0] * ex[1];
                 n_stale += 1;
                 n_vals += ls[h].n;
+                finish(h, f);
             }
         } else {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
+                float f[kFeat];
                 if (c[h] >= 0) {
                     LeafSamples ls;
                     float mn[D], mx[D];
                     e[h] = leaf_error_s<D>(a, c[h], ls, mn, mx);
-                    leaf_features<D>(ls, mn, mx, e[h], depth, a.sX, a.sY, f[h]);
+                    leaf_features<D>(ls, mn, mx, e[h], depth, a.sX, a.sY, f);
                     vol[h] = ls.vol;
                     axy[h] = (mx[0] - mn[0]) * (mx[1] - mn[1]);
                     n_stale += 1;
                     n_vals += ls.n;
                 } else {
 #pragma unroll
-                    for (int k = 0; k < kFeat; ++k) f[h][k] = 0.0f;
+                    for (int k = 0; k < kFeat; ++k) f[k] = 0.0f;
                 }
+                finish(h, f);
             }
         }
-        if (act && !first && feats && l.X != nullptr) {
-            const long long row = rows_base + t;
-            if (row < l.cap_rows) {
-#pragma unroll
-                for (int k = 0; k < kFeat; ++k) l.X[row * kFeat + k] = pf[k];
-                if (l.label_mode == 0) {
-                    l.Y[row] = split_label(pe, e[0], e[1]);
-                } else {
-                    const float mc[2] = {f[0][7], f[1][7]};
-                    l.Y[row] = image_label(pf[7], mc, vol, axy);
-                }
-            }
-        }
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            if (c[h] >= 0 && feats) {
-                if (l.keep) {
-                    float4* fo = (float4*)(l.feat + (size_t)c[h] * kFeat);
-#pragma unroll
-                    for (int k = 0; k < 4; ++k)
-                        fo[k] = make_float4(f[h][4 * k], f[h][4 * k + 1], f[h][4 * k + 2], f[h][4 * k + 3]);
-                    l.perr[c[h]] = e[h];
-                } else {
-                    l.feat[(size_t)c[h] * kFeat + 11] = f[h][11];
-                }
-            }
-        }
+        if (act && !first && feats && l.X != nullptr && rows_base + t < l.cap_rows)
+            l.Y[rows_base + t] = l.label_mode == 0 ? split_label(pe, e[0], e[1])
+                                                   : image_label(pm, mc, vol, axy);
         // scores: the wave's (up to) 128 leaves as 8 tiles of 16 rows
         float q[2] = {e[0], e[1]};
         if (model) {
-            const int hmax = first ? 1 : 2;
-            for (int h = 0; h < hmax; ++h) {
+            for (int tile = 0; tile < 4 * hmax; ++tile) {
+                float y[4];
+                mlp16_bf(l.mlp, w1s, w2s, &xsb[w][16 * tile * kMlpIn], actb[w], y);
+                if ((lane & 15) == 0) {
 #pragma unroll
-                for (int tile = 0; tile < 4; ++tile) {
-                    if ((lane >> 4) == tile) mlp_stage_row(l.mlp, f[h], &xs[w][(lane & 15) * kMlpIn]);
-                    float y[4];
-                    mlp16(l.mlp, xs[w], acts[w], y, nullptr);
-                    if ((lane & 15) == 0) {
+                    for (int i = 0; i < 4; ++i) ys[w][4 * (lane >> 4) + i] = y[i];
+                }
+                wave_sync_lds();
+                const int h = tile >> 2;
+                if ((lane >> 4) == (tile & 3)) {
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) ys[w][4 * (lane >> 4) + i] = y[i];
-                    }
-                    wave_sync_lds();
-                    if ((lane >> 4) == tile && c[h] >= 0) {
+                    for (int hh = 0; hh < 2; ++hh) {
+                        if (hh != h || c[hh] < 0) continue;
                         const float yv = ys[w][lane & 15];
                         if (fabsf(yv) < 3.0e38f) {  // false for NaN and +-inf
                             const float yc = fminf(fmaxf(yv, l.ylo), l.yhi);
                             float s;
                             if (l.kappa_e == 1.0f && l.kappa_y == 1.0f)
-                                s = e[h] * exp2f(yc);
+                                s = e[hh] * exp2f(yc);
                             else if (l.kappa_e == 0.0f && l.kappa_y == 1.0f)
                                 s = exp2f(yc);
                             else  // e^kappa_e * 2^(kappa_y y)
-                                s = exp2f(l.kappa_e * log2_floor(e[h]) + l.kappa_y * yc);
-                            if (s < 3.0e38f) q[h] = s;
+                                s = exp2f(l.kappa_e * log2_floor(e[hh]) + l.kappa_y * yc);
+                            if (s < 3.0e38f) q[hh] = s;
                         }
                     }
-                    wave_sync_lds();
                 }
+                wave_sync_lds();
             }
         }
 #pragma unroll
@@ -278,12 +293,16 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
     }
 }
 
+// 4 waves per SIMD: the feature phase and the network fit the register file
+// at that occupancy without spilling (the default allocation gave 3).
 template <int D>
-__global__ __launch_bounds__(256) void k_errors_learned(DevGrid a, LearnDev l) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_errors_learned(
+    DevGrid a, LearnDev l) {
     errors_learned_body<D>(a, l, blockIdx.x, gridDim.x);
 }
 template <int D>
-__global__ __launch_bounds__(256) void k_errors_learned_b(const DevGrid* grids, const LearnDev* ls) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_errors_learned_b(
+    const DevGrid* grids, const LearnDev* ls) {
     errors_learned_body<D>(grids[blockIdx.y], ls[blockIdx.y], blockIdx.x, gridDim.x);
 }
 

@@ -164,4 +164,69 @@ __device__ __forceinline__ void mlp16(const MlpDev& M, const float* x, float* ac
     wave_sync_lds();
 }
 
+// The same forward pass from rows staged as bf16 bits (x: LDS [16][16]) with
+// the bf16 layer-1 activations kept as bf16 bits (act: LDS [16][64]): the
+// values are those of mlp16 (its rows and activations are rounded to bf16
+// before use), in half the LDS; the bf16 weights are read from the
+// workgroup's LDS copies w1s [64][32] and w2s [64][64].  Inference only.
+__device__ __forceinline__ void mlp16_bf(const MlpDev& M, const unsigned short* w1s,
+                                         const unsigned short* w2s, const unsigned short* x,
+                                         unsigned short* act, float y[4]) {
+    const int lane = threadIdx.x & 63;
+    const int col = lane & 15, kg = lane >> 4;
+    wave_sync_lds();
+    u16x8 av;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int kk = 8 * kg + k;
+        av[k] = kk < kMlpIn ? x[col * kMlpIn + kk] : (unsigned short)0;
+    }
+    const bf16x8 a1 = __builtin_bit_cast(bf16x8, av);
+    f32x4 z[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const bf16x8 b = load_b8(w1s + (16 * t + col) * 32 + 8 * kg);
+        const f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+        z[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b, c, 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int n = 16 * t + col;
+        const float bias = M.p[kOffB1 + n];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) act[(4 * kg + i) * kMlpH + n] = f2bf(gelu(z[t][i] + bias));
+    }
+    wave_sync_lds();
+    bf16x8 a2[2];
+#pragma unroll
+    for (int kc = 0; kc < 2; ++kc) {
+        u16x8 v;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = act[col * kMlpH + 32 * kc + 8 * kg + k];
+        a2[kc] = __builtin_bit_cast(bf16x8, v);
+    }
+    float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kc = 0; kc < 2; ++kc) {
+            const bf16x8 b = load_b8(w2s + (16 * t + col) * 64 + 32 * kc + 8 * kg);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[kc], b, c, 0, 0, 0);
+        }
+        const int n = 16 * t + col;
+        const float bias = M.p[kOffB2 + n], w3 = M.w3r[n];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) part[i] += bfround(gelu(c[i] + bias)) * w3;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float s = part[i];
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) s += __shfl_xor(s, o, 64);
+        y[i] = s + M.p[kOffB3];
+    }
+    wave_sync_lds();
+}
+
 }  // namespace mdas
