@@ -698,6 +698,31 @@ void orc_eval_scene_f64(const void* params, const double* coords, int dim, long 
     }
 }
 
+void orc_eval_scene_grouped(const void* params, const void* coords, int dim, long long count,
+                            int fp64, void* values) {
+    const mscene::Params& P = *(const mscene::Params*)params;
+    mscene::Accel A;
+    mscene::build_accel(P, &A);
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(nthreads())
+    for (long long k = 0; k < count; ++k) {
+        if (fp64) {
+            const mscene::V3<double> v =
+                mscene::eval<double>(P, P.obj, (const double*)coords + k * dim, &A);
+            double* o = (double*)values + 3 * k;
+            o[0] = v.x;
+            o[1] = v.y;
+            o[2] = v.z;
+        } else {
+            const mscene::V3<float> v =
+                mscene::eval<float>(P, P.obj, (const float*)coords + k * dim, &A);
+            float* o = (float*)values + 3 * k;
+            o[0] = v.x;
+            o[1] = v.y;
+            o[2] = v.z;
+        }
+    }
+}
+
 float orc_halton(int dim, int i) { return halton<float>(dim, i); }
 
 void orc_qmc_image(const void* params, int w, int h, int spp, int first, uint32_t frame, int fp64,

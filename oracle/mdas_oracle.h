@@ -52,6 +52,13 @@ void orc_eval_scene(const void* scene_params, const float* coords, int dim, long
 void orc_eval_range(orc_grid* g, const void* scene_params, long long first, long long count);
 void orc_eval_scene_f64(const void* scene_params, const double* coords, int dim,
                         long long count, double* values);
+/* The same integrand through the product's sphere groups (mscene::build_accel,
+ * the pruned loops of mscene::trace / occluded), compiled here so that a CPU
+ * test can hold it against the plain loops above: float coords/values
+ * (fp64 = 0) or double (fp64 = 1).  A check of the product's pruning, not a
+ * restatement of the reference. */
+void orc_eval_scene_grouped(const void* scene_params, const void* coords, int dim,
+                            long long count, int fp64, void* values);
 /* The plain QMC sampler's image (cuda/sampler.h:4-28, random.h:74-89,
  * whitted.cu:59-69): float4 pixels (fp64 = 0) or double4 (fp64 = 1). */
 void orc_qmc_image(const void* scene_params, int w, int h, int spp, int first_sample,

@@ -35,6 +35,7 @@ _SIGS = {
     "orc_make_scene": (_LL, [_I, _U, _I, _F, _F, _P, _LL]),
     "orc_eval_scene": (None, [_P, _P, _I, _LL, _LL, _P]),
     "orc_eval_scene_f64": (None, [_P, _P, _I, _LL, _P]),
+    "orc_eval_scene_grouped": (None, [_P, _P, _I, _LL, _I, _P]),
     "orc_qmc_image": (None, [_P, _I, _I, _I, _I, _U, _I, _P]),
     "orc_halton": (_F, [_I, _I]),
     "orc_eval_range": (None, [_P, _P, _LL, _LL]),
@@ -220,6 +221,16 @@ def eval_scene_f64(params, coords, dim):
     c = np.ascontiguousarray(coords, np.float64).reshape(-1, dim)
     out = np.empty((len(c), 3), np.float64)
     load().orc_eval_scene_f64(params, vp(c), dim, len(c), vp(out))
+    return out
+
+
+def eval_scene_grouped(params, coords, dim, fp64=False):
+    """The integrand through the product's sphere groups (CPU build), to be
+    held against eval_scene / eval_scene_f64."""
+    t = np.float64 if fp64 else np.float32
+    c = np.ascontiguousarray(coords, t).reshape(-1, dim)
+    out = np.empty((len(c), 3), t)
+    load().orc_eval_scene_grouped(params, vp(c), dim, len(c), 1 if fp64 else 0, vp(out))
     return out
 
 

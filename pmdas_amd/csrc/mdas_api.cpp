@@ -591,8 +591,13 @@ int mdas_scene_create(int kind, uint32_t seed, int dim, float sx, float sy, mdas
     if (dim < need[kind] || dim > 6) return fail(MDAS_ERR_INVALID, "scene needs more dimensions");
     mdas_scene* s = new mdas_scene;
     mscene::make_scene(&s->host, kind, seed, dim, sx, sy);
-    hipError_t e = hipMalloc(&s->dev, sizeof(mscene::Params));
-    if (e == hipSuccess) e = hipMemcpy(s->dev, &s->host, sizeof(mscene::Params), hipMemcpyHostToDevice);
+    // the device copy: the parameters followed by the sphere groups
+    std::vector<mscene::SceneBlock> blk(1);
+    blk[0].p = s->host;
+    mscene::build_accel(s->host, &blk[0].accel);
+    hipError_t e = hipMalloc(&s->dev, sizeof(mscene::SceneBlock));
+    if (e == hipSuccess)
+        e = hipMemcpy(s->dev, blk.data(), sizeof(mscene::SceneBlock), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         if (s->dev) hipFree(s->dev);
         delete s;

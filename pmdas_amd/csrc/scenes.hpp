@@ -175,20 +175,90 @@ MDAS_HD void camera_ray(const Params& P, real dx, real dy, real lu, real lv, boo
     *d = normalize3(dir);
 }
 
+// ------------------------------------------------ grouped sphere tests --
+// The spheres of a scene in groups of up to GRP_SIZE (median splits of their
+// centres), each group with a bounding sphere (centre Gc, radius R) that
+// holds its members over the whole shutter t in [0, 1].  A ray skips a group
+// only where no member's own test (hit_sphere, in the same arithmetic) can
+// report a hit that counts, so the closest hit over the other members, ties
+// to the lowest index, is the one a loop over all spheres finds, bit for bit.
+//
+// The bound.  With X = o - centre, the member test finds a root when
+// (X.d)^2 - |X|^2 + r^2 >= 0 in rounded arithmetic.  Writing |d|^2 = 1 + e,
+// |X|^2 - (X.d)^2 = |X_perp|^2 - e X_par^2, and the rounding of that
+// expression stays below 2e-6 |X|^2 (about 13 unit roundoffs), so a member
+// can report a root only if |X_perp| <= r + sqrt(G) with
+// G = (2e-6 + |e|) (|o - Gc| + R)^2; its roots then lie within r + sqrt(G)
+// of its centre's projection, which is within R - r of the group's.  The group
+// test widens R by S = sqrt(G) (with G taken as (2e-6 + |e|) 2 (l2 + R^2),
+// l2 = |o - Gc|^2), a further |e| R + kCullSlack, and discounts its own
+// rounding and the direction's length from the squared line distance.
+static const int MAX_GRP = 64;
+static const int GRP_SIZE = 8;
+constexpr float kCullSlack = 0.01f;
+
+struct Accel {
+    int32_t n_grp;               // 0: no groups, every sphere is tested
+    int32_t pad[3];
+    float grp[MAX_GRP * 4];      // bounding sphere of each group: centre, radius
+    int32_t range[MAX_GRP * 2];  // first member position and member count
+    int32_t member[MAX_OBJ];     // sphere indices, group by group
+};
+
+// A scene as the integrand kernels read it: the parameters, then the groups.
+struct SceneBlock {
+    Params p;
+    Accel accel;
+};
+
+// Whether a member of group g may report a hit in (tmin, tmax) for the ray
+// (o, d); e = |d.d - 1|.
+template <typename real>
+MDAS_HD bool group_may_hit(const float* g, V3<real> o, V3<real> d, real e, real tmin,
+                           real tmax) {
+    const real gx = (real)g[0] - o.x, gy = (real)g[1] - o.y, gz = (real)g[2] - o.z;
+    const real R = (real)g[3];
+    const real l2 = gx * gx + gy * gy + gz * gz;
+    const real p = gx * d.x + gy * d.y + gz * d.z;  // centre's distance along the ray
+    const real k = (real)2e-6 + e;
+    const real S = r_sqrt(k * (real)2 * (l2 + R * R)) + e * R + (real)kCullSlack;
+    const real W = R + S;
+    return l2 - p * p - k * l2 <= W * W && p + W >= tmin && p - W <= tmax;
+}
+
 // Closest hit among spheres (centres moved to time t) and the ground plane.
 // Returns the hit distance (tmax on a miss); *id = sphere index, -1 plane, -2 miss.
+// With groups (A != null, A->n_grp > 0) the spheres of the groups the ray can
+// meet are tested group by group; the result is the same.
 template <typename real>
 MDAS_HD real trace(const Params& P, const float* obj, V3<real> o, V3<real> d, real t,
-                   real tmin, real tmax, int* id) {
+                   real tmin, real tmax, int* id, const Accel* A = nullptr) {
     real best = tmax;
     *id = -2;
-    for (int k = 0; k < P.n_obj; ++k) {
-        const float* s = obj + k * OBJ_STRIDE;
-        real cx = (real)s[0] + (real)s[4] * t;
-        real cy = (real)s[1] + (real)s[5] * t;
-        real cz = (real)s[2] + (real)s[6] * t;
-        real th = hit_sphere(o, d, cx, cy, cz, (real)s[3], tmin, best);
-        if (th < best) { best = th; *id = k; }
+    if (A != nullptr && A->n_grp > 0) {
+        const real e = r_abs(dot3(d, d) - (real)1);
+        for (int g = 0; g < A->n_grp; ++g) {
+            if (!group_may_hit(A->grp + 4 * g, o, d, e, tmin, best)) continue;
+            const int j0 = A->range[2 * g], j1 = j0 + A->range[2 * g + 1];
+            for (int j = j0; j < j1; ++j) {
+                const int k = A->member[j];
+                const float* s = obj + k * OBJ_STRIDE;
+                real cx = (real)s[0] + (real)s[4] * t;
+                real cy = (real)s[1] + (real)s[5] * t;
+                real cz = (real)s[2] + (real)s[6] * t;
+                real th = hit_sphere(o, d, cx, cy, cz, (real)s[3], tmin, tmax);
+                if (th < best || (th == best && th < tmax && k < *id)) { best = th; *id = k; }
+            }
+        }
+    } else {
+        for (int k = 0; k < P.n_obj; ++k) {
+            const float* s = obj + k * OBJ_STRIDE;
+            real cx = (real)s[0] + (real)s[4] * t;
+            real cy = (real)s[1] + (real)s[5] * t;
+            real cz = (real)s[2] + (real)s[6] * t;
+            real th = hit_sphere(o, d, cx, cy, cz, (real)s[3], tmin, best);
+            if (th < best) { best = th; *id = k; }
+        }
     }
     if (d.y != (real)0) {
         real tp = ((real)P.plane_y - o.y) / d.y;
@@ -197,16 +267,32 @@ MDAS_HD real trace(const Params& P, const float* obj, V3<real> o, V3<real> d, re
     return best;
 }
 
-// Any hit in (tmin, tmax) among spheres and the plane.
+// Any hit in (tmin, tmax) among spheres and the plane (with groups: among
+// the spheres of the groups the ray can meet, the same answer).
 template <typename real>
 MDAS_HD bool occluded(const Params& P, const float* obj, V3<real> o, V3<real> d, real t,
-                      real tmin, real tmax, bool with_plane) {
-    for (int k = 0; k < P.n_obj; ++k) {
-        const float* s = obj + k * OBJ_STRIDE;
-        real cx = (real)s[0] + (real)s[4] * t;
-        real cy = (real)s[1] + (real)s[5] * t;
-        real cz = (real)s[2] + (real)s[6] * t;
-        if (hit_sphere(o, d, cx, cy, cz, (real)s[3], tmin, tmax) < tmax) return true;
+                      real tmin, real tmax, bool with_plane, const Accel* A = nullptr) {
+    if (A != nullptr && A->n_grp > 0) {
+        const real e = r_abs(dot3(d, d) - (real)1);
+        for (int g = 0; g < A->n_grp; ++g) {
+            if (!group_may_hit(A->grp + 4 * g, o, d, e, tmin, tmax)) continue;
+            const int j0 = A->range[2 * g], j1 = j0 + A->range[2 * g + 1];
+            for (int j = j0; j < j1; ++j) {
+                const float* s = obj + A->member[j] * OBJ_STRIDE;
+                real cx = (real)s[0] + (real)s[4] * t;
+                real cy = (real)s[1] + (real)s[5] * t;
+                real cz = (real)s[2] + (real)s[6] * t;
+                if (hit_sphere(o, d, cx, cy, cz, (real)s[3], tmin, tmax) < tmax) return true;
+            }
+        }
+    } else {
+        for (int k = 0; k < P.n_obj; ++k) {
+            const float* s = obj + k * OBJ_STRIDE;
+            real cx = (real)s[0] + (real)s[4] * t;
+            real cy = (real)s[1] + (real)s[5] * t;
+            real cz = (real)s[2] + (real)s[6] * t;
+            if (hit_sphere(o, d, cx, cy, cz, (real)s[3], tmin, tmax) < tmax) return true;
+        }
     }
     if (with_plane && d.y != (real)0) {
         real tp = ((real)P.plane_y - o.y) / d.y;
@@ -218,10 +304,10 @@ MDAS_HD bool occluded(const Params& P, const float* obj, V3<real> o, V3<real> d,
 // Lambertian shading of a directional-light scene (DOF and MB+DOF scenes).
 template <typename real>
 MDAS_HD V3<real> shade_spheres(const Params& P, const float* obj, V3<real> o, V3<real> d,
-                               real t) {
+                               real t, const Accel* A) {
     int id;
     const real big = (real)1e30;
-    real th = trace(P, obj, o, d, t, (real)1e-3, big, &id);
+    real th = trace(P, obj, o, d, t, (real)1e-3, big, &id, A);
     if (id == -2) {
         real a = (real)0.5 * (d.y + (real)1);
         return v3((real)1 - a * (real)0.5, (real)1 - a * (real)0.3, (real)1);
@@ -247,7 +333,7 @@ MDAS_HD V3<real> shade_spheres(const Params& P, const float* obj, V3<real> o, V3
     if (ndl > (real)0) {
         real eps = (real)1e-3;
         V3<real> so = v3(p.x + eps * n.x, p.y + eps * n.y, p.z + eps * n.z);
-        if (!occluded(P, obj, so, l, t, (real)0, big, false)) direct = ndl;
+        if (!occluded(P, obj, so, l, t, (real)0, big, false, A)) direct = ndl;
     }
     real amb = (real)P.ambient;
     return v3(ar * (amb + direct * (real)P.light_color[0]),
@@ -255,9 +341,11 @@ MDAS_HD V3<real> shade_spheres(const Params& P, const float* obj, V3<real> o, V3
               ab * (amb + direct * (real)P.light_color[2]));
 }
 
-// Evaluate one sample.  `x` holds P.dim coordinates.
+// Evaluate one sample.  `x` holds P.dim coordinates; A: the scene's sphere
+// groups, or null to test every sphere.
 template <typename real>
-MDAS_HD V3<real> eval(const Params& P, const float* obj, const real* x) {
+MDAS_HD V3<real> eval(const Params& P, const float* obj, const real* x,
+                      const Accel* A = nullptr) {
     real dx = (real)2 * (x[0] / (real)P.scale_x) - (real)1;
     real dy = (real)2 * (x[1] / (real)P.scale_y) - (real)1;
     switch (P.kind) {
@@ -285,19 +373,19 @@ MDAS_HD V3<real> eval(const Params& P, const float* obj, const real* x) {
     case SCENE_SPHERES_DOF: {
         V3<real> o, d;
         camera_ray(P, dx, dy, x[2], x[3], true, &o, &d);
-        return shade_spheres(P, obj, o, d, (real)0);
+        return shade_spheres(P, obj, o, d, (real)0, A);
     }
     case SCENE_SPHERES_MBDOF: {
         V3<real> o, d;
         camera_ray(P, dx, dy, x[3], x[4], true, &o, &d);
-        return shade_spheres(P, obj, o, d, x[2]);
+        return shade_spheres(P, obj, o, d, x[2], A);
     }
     case SCENE_AO: {
         V3<real> o, d;
         camera_ray(P, dx, dy, x[2], x[3], true, &o, &d);
         int id;
         const real big = (real)1e30;
-        real th = trace(P, obj, o, d, (real)0, (real)1e-3, big, &id);
+        real th = trace(P, obj, o, d, (real)0, (real)1e-3, big, &id, A);
         if (id == -2) return v3((real)1, (real)1, (real)1);
         V3<real> p = v3(o.x + th * d.x, o.y + th * d.y, o.z + th * d.z);
         V3<real> n;
@@ -324,7 +412,7 @@ MDAS_HD V3<real> eval(const Params& P, const float* obj, const real* x) {
                         hx * t1.z + hy * t2.z + hz * n.z);
         real eps = (real)1e-3;
         V3<real> so = v3(p.x + eps * n.x, p.y + eps * n.y, p.z + eps * n.z);
-        bool occ = occluded(P, obj, so, w, (real)0, (real)0, (real)P.ao_radius, true);
+        bool occ = occluded(P, obj, so, w, (real)0, (real)0, (real)P.ao_radius, true, A);
         real vis = occ ? (real)0 : (real)1;
         return v3(vis, vis, vis);
     }
@@ -412,6 +500,112 @@ inline void make_scene(Params* P, int kind, uint32_t seed, int dim, float scale_
     default:
         break;
     }
+}
+
+// Groups of a sphere scene (none for the others): each sphere as its sweep
+// over t in [0, 1] (centre c + v/2, radius r + |v|/2), split at the median
+// along the longest extent of the sweep centres until at most GRP_SIZE
+// remain, groups ordered by the depth (z) of their centre so that a camera
+// ray, which looks along +z, meets the near ones first.  Bounds in double,
+// the radius rounded up.
+inline void build_accel(const Params& P, Accel* A) {
+    char* raw = (char*)A;
+    for (unsigned i = 0; i < sizeof(Accel); ++i) raw[i] = 0;
+    if (P.kind != SCENE_SPHERES_DOF && P.kind != SCENE_SPHERES_MBDOF && P.kind != SCENE_AO) return;
+    const int n = P.n_obj;
+    if (n <= 0 || n > MAX_OBJ) return;
+    double mid[MAX_OBJ][3], ext[MAX_OBJ];
+    int idx[MAX_OBJ];
+    for (int k = 0; k < n; ++k) {
+        const float* s = P.obj + k * OBJ_STRIDE;
+        double vv = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            mid[k][a] = (double)s[a] + 0.5 * (double)s[4 + a];
+            vv += (double)s[4 + a] * (double)s[4 + a];
+        }
+        ext[k] = (double)s[3] + 0.5 * __builtin_sqrt(vv);
+        idx[k] = k;
+    }
+    int sb[64], se[64], sp = 0, ng = 0;
+    int gb[MAX_GRP], ge[MAX_GRP];
+    sb[sp] = 0;
+    se[sp++] = n;
+    while (sp > 0) {
+        const int b = sb[--sp], e = se[sp];
+        if (e - b <= GRP_SIZE) {
+            gb[ng] = b;
+            ge[ng++] = e;
+            continue;
+        }
+        int axis = 0;
+        double best = -1.0;
+        for (int a = 0; a < 3; ++a) {
+            double lo = mid[idx[b]][a], hi = lo;
+            for (int j = b + 1; j < e; ++j) {
+                lo = mid[idx[j]][a] < lo ? mid[idx[j]][a] : lo;
+                hi = mid[idx[j]][a] > hi ? mid[idx[j]][a] : hi;
+            }
+            if (hi - lo > best) { best = hi - lo; axis = a; }
+        }
+        for (int j = b + 1; j < e; ++j) {  // insertion sort by (coordinate, index)
+            const int v = idx[j];
+            int q = j - 1;
+            while (q >= b && (mid[idx[q]][axis] > mid[v][axis] ||
+                              (mid[idx[q]][axis] == mid[v][axis] && idx[q] > v))) {
+                idx[q + 1] = idx[q];
+                --q;
+            }
+            idx[q + 1] = v;
+        }
+        const int h = (b + e) / 2;
+        sb[sp] = h;
+        se[sp++] = e;
+        sb[sp] = b;
+        se[sp++] = h;
+    }
+    float gc[MAX_GRP][4];
+    for (int g = 0; g < ng; ++g) {
+        double lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = mid[idx[gb[g]]][a] - ext[idx[gb[g]]];
+            hi[a] = mid[idx[gb[g]]][a] + ext[idx[gb[g]]];
+            for (int j = gb[g] + 1; j < ge[g]; ++j) {
+                const double l = mid[idx[j]][a] - ext[idx[j]], u = mid[idx[j]][a] + ext[idx[j]];
+                lo[a] = l < lo[a] ? l : lo[a];
+                hi[a] = u > hi[a] ? u : hi[a];
+            }
+            gc[g][a] = (float)(0.5 * (lo[a] + hi[a]));
+        }
+        double R = 0.0;
+        for (int j = gb[g]; j < ge[g]; ++j) {
+            double d2 = 0.0;
+            for (int a = 0; a < 3; ++a) {
+                const double t = mid[idx[j]][a] - (double)gc[g][a];
+                d2 += t * t;
+            }
+            const double r = __builtin_sqrt(d2) + ext[idx[j]];
+            R = r > R ? r : R;
+        }
+        gc[g][3] = (float)(R * (1.0 + 1e-6) + 1e-6);
+    }
+    int order[MAX_GRP];
+    for (int g = 0; g < ng; ++g) {  // by centre depth, ties in split order
+        int q = g - 1;
+        while (q >= 0 && gc[order[q]][2] > gc[g][2]) {
+            order[q + 1] = order[q];
+            --q;
+        }
+        order[q + 1] = g;
+    }
+    int pos = 0;
+    for (int r = 0; r < ng; ++r) {
+        const int g = order[r];
+        for (int a = 0; a < 4; ++a) A->grp[4 * r + a] = gc[g][a];
+        A->range[2 * r] = pos;
+        A->range[2 * r + 1] = ge[g] - gb[g];
+        for (int j = gb[g]; j < ge[g]; ++j) A->member[pos++] = idx[j];
+    }
+    A->n_grp = ng;
 }
 
 }  // namespace mscene
