@@ -87,6 +87,7 @@ def run_gpu(name, steps, warmup, log=print):
         log(f"[bench] {name}: step {i + 1}/{steps}: {mdas_ms:.2f} ms MDAS, "
             f"{ph['integrand_ms']:.2f} ms integrand, {(t1 - t0) * 1e3:.2f} ms wall, "
             f"{ph['passes']} passes, N={g.GetNumberOfSamples()}")
+    pstats = g.pass_stats()
     # the same steps with kernel timing off (no per-kernel events), as the
     # library runs in production: the end-to-end rate
     g.set_timing(False)
@@ -109,6 +110,8 @@ def run_gpu(name, steps, warmup, log=print):
         kernels={k: dict(launches=v[0] // steps, ms=v[1] / steps, bytes=v[2] / steps)
                  for k, v in kstats.items()},
     )
+    res["phase_model_bytes"] = phase_model_bytes(w, pstats, res["leaves"], res["samples"],
+                                                 res["kernels"].get("k_rs_scatter"))
     px.free()
     g.close()
     sc.close()
@@ -324,6 +327,35 @@ def run_frames(log=print):
             "train_seconds": round(t_train, 1), "recon_tile": tile}
 
 
+def phase_model_bytes(w, pstats, L, N, scatter):
+    """Bytes of each MDAS phase as the reference's kernels move them, from the
+    per-pass counters (rows of L, N, selected, split, samples of the selected
+    leaves, re-estimated leaves, values they read): initial sampling writes
+    4 points, node, box, error and seed per leaf (kdgrid.cu:121-202); the
+    error pass reads every error and, per re-estimated leaf, node, box and
+    values and writes its error (:204-288); the adaptive pass reads every
+    leaf's error, box and seed and writes the seed, per selected leaf its
+    node, its samples' points, and writes the new point and node, per split
+    a node, two boxes and two errors (:290-481); the reconstruction reads
+    node, box and values per leaf and writes every pixel (:483-582); the
+    sort reads and writes key and payload per pass (12 B each way)."""
+    D = w["dim"]
+    L0 = 1 << (w["b"] * D + 2 * w["e"])
+    rows = np.asarray(pstats, dtype=np.float64)
+    Lp, sel, split, selcnt, stale, svals = (rows[:, 0], rows[:, 2], rows[:, 3], rows[:, 4],
+                                            rows[:, 5], rows[:, 6])
+    b = {
+        "initial": L0 * (16 * D + 16 + 8 * D + 8),
+        "errors": float(np.sum(4 * Lp + stale * (16 + 8 * D + 4) + 12 * svals)),
+        "adaptive": float(np.sum(Lp * (4 + 8 * D + 8) + sel * (16 + 4 * D + 16) + 4 * D * selcnt
+                                 + split * (16 + 16 * D + 8))),
+        "recon": float(L * (16 + 8 * D) + 12 * N + 16 * w["W"] * w["H"]),
+    }
+    if scatter:  # key passes = scatter bytes / 16 (8-byte keys read and written)
+        b["recon"] += 24.0 * scatter["bytes"] / 16.0
+    return b
+
+
 def roofline(res, traffic_file):
     """Dominant MDAS kernel (integrand excluded): algorithmic bytes / its time."""
     ks = {k: v for k, v in res["kernels"].items() if k != "k_eval"}
@@ -336,10 +368,25 @@ def roofline(res, traffic_file):
             t = json.load(f)
         if t.get("kernel") == name:
             traffic = t.get("hbm_bytes_per_launch")
-    return dict(bound="hbm", kernel=name, achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
-                unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                algorithmic_bytes_per_launch=k["bytes"] / max(k["launches"], 1),
-                avg_launch_ms=k["ms"] / max(k["launches"], 1), launches=k["launches"])
+    out = dict(bound="hbm", kernel=name, achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
+               unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+               algorithmic_bytes_per_launch=k["bytes"] / max(k["launches"], 1),
+               avg_launch_ms=k["ms"] / max(k["launches"], 1), launches=k["launches"])
+    # the whole run under the reference kernels' per-phase bytes: what those
+    # kernels would move, over this build's MDAS phase time
+    pb = res.get("phase_model_bytes")
+    if pb:
+        ph = res["phases_ms"]
+        tot = sum(pb.values())
+        gbs = tot / (res["mdas_ms"] * 1e-3) / 1e9
+        out["whole_run"] = dict(
+            bytes=round(tot), ms=round(res["mdas_ms"], 3), achieved=round(gbs, 1),
+            frac=round(gbs / HBM_PEAK_GBS, 4),
+            phases={k: dict(bytes=round(v), GBps=round(v / (ph[k + "_ms"] * 1e-3) / 1e9, 1))
+                    for k, v in pb.items()},
+            note="reference kernels' bytes per phase (per-leaf error/box/seed sweeps, "
+                 "node/box/value reads) over this build's phase times")
+    return out
 
 
 def cpu_baseline(name, shrink=0, log=print):
