@@ -356,7 +356,22 @@ def phase_model_bytes(w, pstats, L, N, scatter):
     return b
 
 
-def roofline(res, traffic_file):
+def copy_probe():
+    """Achievable HBM bandwidth of this GPU: best of 5 plain float4 copies of
+    1 GiB (pmdas_amd/bw_probe, read + write bytes / time), run as a child
+    process before this one touches the GPU; None if the probe is not built."""
+    exe = os.path.join(ROOT, "pmdas_amd", "bw_probe")
+    if not os.path.exists(exe):
+        return None
+    import subprocess
+    r = subprocess.run([exe, "1024"], capture_output=True, text=True, timeout=120)
+    for ln in r.stdout.splitlines():
+        if r.returncode == 0 and ln.startswith("best:"):
+            return float(ln.split()[1])
+    return None
+
+
+def roofline(res, traffic_file, copy_gbs=None):
     """Dominant MDAS kernel (integrand excluded): algorithmic bytes / its time."""
     ks = {k: v for k, v in res["kernels"].items() if k != "k_eval"}
     name = max(ks, key=lambda k: ks[k]["ms"])
@@ -371,7 +386,9 @@ def roofline(res, traffic_file):
     out = dict(bound="hbm", kernel=name, achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
                unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                algorithmic_bytes_per_launch=k["bytes"] / max(k["launches"], 1),
-               avg_launch_ms=k["ms"] / max(k["launches"], 1), launches=k["launches"])
+               avg_launch_ms=k["ms"] / max(k["launches"], 1), launches=k["launches"],
+               copy_probe_GBps=copy_gbs,
+               frac_of_copy=round(achieved / copy_gbs, 4) if copy_gbs else None)
     # the whole run under the reference kernels' per-phase bytes: what those
     # kernels would move, over this build's MDAS phase time
     pb = res.get("phase_model_bytes")
@@ -382,6 +399,7 @@ def roofline(res, traffic_file):
         out["whole_run"] = dict(
             bytes=round(tot), ms=round(res["mdas_ms"], 3), achieved=round(gbs, 1),
             frac=round(gbs / HBM_PEAK_GBS, 4),
+            frac_of_copy=round(gbs / copy_gbs, 4) if copy_gbs else None,
             phases={k: dict(bytes=round(v), GBps=round(v / (ph[k + "_ms"] * 1e-3) / 1e9, 1))
                     for k, v in pb.items()},
             note="reference kernels' bytes per phase (per-leaf error/box/seed sweeps, "
@@ -454,6 +472,8 @@ def main():
               "sharding by image or screen tile is a design, not built)", file=sys.stderr)
         sys.exit(2)
     log = lambda m: print(m, file=sys.stderr, flush=True)
+    copy_gbs = copy_probe()  # before this process initialises the GPU
+    log(f"[bench] copy probe: {copy_gbs} GB/s")
     import pmdas_amd
     from pmdas_amd import hip
     pmdas_amd.load()
@@ -490,7 +510,7 @@ def main():
         "wall_ms_untimed": round(res["wall_ms_untimed"], 3),
         "samples": res["samples"], "leaves": res["leaves"], "passes": res["passes"],
         "phases_ms": {k: round(v, 3) for k, v in res["phases_ms"].items()},
-        "roofline": roofline(res, traffic_file),
+        "roofline": roofline(res, traffic_file, copy_gbs),
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 4),
                         "GBps": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)}
                     for k, v in sorted(res["kernels"].items(), key=lambda kv: -kv[1]["ms"])},
