@@ -135,7 +135,7 @@ def batch_images(n=32, seed=7):
 
 
 def run_batch(steps, log=print):
-    """Config 3: 32 images of uneven resolution, 5D, 32 spp, advanced as one
+    """The batch workload: 32 images of uneven resolution, 5D, 32 spp, advanced as one
     batch replayed from a captured hipGraph, then reconstructed one by one
     with the gather tile the autotuner picked for each image in the warmup."""
     from pmdas_amd import hip, kdgrid
