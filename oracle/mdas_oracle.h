@@ -40,7 +40,7 @@ void orc_convert_to_bytes(const float* pixels, unsigned char* out, long long n);
 void orc_sampling_density(orc_grid* g, float* pixels, int w, int h); /* kdgrid.cu:822-837 */
 int orc_validate(orc_grid* g);                    /* kdgrid.cu:848-961, always on */
 
-/* Whole run with the driver's stopping rule (optixMotionBlur.cpp:768-791),
+/* Whole run with the sample app's stopping rule (optixMotionBlur.cpp:768-791),
  * using the analytic scene `params` (mdas_scene::Params layout) as integrand. */
 int orc_run(orc_grid* g, const void* scene_params, long long target_samples, int* passes);
 /* Scene parameter block (same generator as the product's mdas_scene_create). */

@@ -1,6 +1,6 @@
 """bench.py's command-line contract on a host without a GPU: more than one
 GPU is refused with a message (the sampler runs on one GPU), and the help
-text names the flags the benchmark driver passes."""
+text names the flags a benchmark harness passes (--gpus, --steps, --warmup)."""
 import os
 import subprocess
 import sys
@@ -20,7 +20,7 @@ def test_more_than_one_gpu_is_refused():
     assert not r.stdout.strip().startswith("{")  # no result line
 
 
-def test_help_names_the_driver_flags():
+def test_help_names_the_harness_flags():
     r = _bench("--help")
     assert r.returncode == 0
     for flag in ("--gpus", "--steps", "--warmup"):

@@ -134,7 +134,7 @@ def test_pass_by_pass_bit_exact(mdas, kind, dim, b, e, W, H, spp, alpha, cand):
 
 @pytest.mark.parametrize("kind,dim,b,e,W,H,spp,alpha,cand", RUNS)
 def test_run_loop_bit_exact(mdas, kind, dim, b, e, W, H, spp, alpha, cand):
-    """mdas_run (device-side stopping rule) == the oracle's driver loop."""
+    """mdas_run (device-side stopping rule) == the oracle's sampling loop."""
     _say(f"run kind={kind} dim={dim} {W}x{H} spp={spp}")
     g, o, sc, target = _make(mdas, kind, dim, b, e, W, H, spp, alpha, cand)
     passes = g.Run(sc, target)
