@@ -407,11 +407,11 @@ def roofline(res, traffic_file, copy_gbs=None):
     return out
 
 
-def cpu_baseline(name, shrink=0, log=print):
+def cpu_baseline(name, shrink=0, log=print, runs=3):
     """The OpenMP oracle on the host cores, MDAS phases only (integrand
-    excluded, as on the GPU).  shrink = k runs a (W/2^k) x (H/2^k) version of
-    the workload with extraImgBits - k, which keeps 4 initial samples per
-    initial cell and the same spp."""
+    excluded, as on the GPU), the median of `runs` runs.  shrink = k runs a
+    (W/2^k) x (H/2^k) version of the workload with extraImgBits - k, which
+    keeps 4 initial samples per initial cell and the same spp."""
     from oracle import pyoracle
     w = dict(WORKLOADS[name])
     w["W"] >>= shrink
@@ -422,35 +422,40 @@ def cpu_baseline(name, shrink=0, log=print):
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     threads = min(threads, 16)
     lib.orc_set_threads(threads)
-    o = pyoracle.OracleGrid(w["dim"], cap, w["cand"], w["b"], w["e"], w["alpha"], float(w["W"]),
-                            float(w["H"]))
     params = pyoracle.make_scene(w["kind"], w["seed"], w["dim"], float(w["W"]), float(w["H"]))
-    t_mdas = 0.0
-    t0 = time.perf_counter()
-    o.initial_sampling()
-    t_mdas += time.perf_counter() - t0
-    o.eval_scene(params, 0, o.n_samples)
-    passes = 0
-    while o.n_samples < target:
+    times = []
+    for r in range(runs):
+        o = pyoracle.OracleGrid(w["dim"], cap, w["cand"], w["b"], w["e"], w["alpha"],
+                                float(w["W"]), float(w["H"]))
+        t_mdas = 0.0
         t0 = time.perf_counter()
-        o.compute_errors()
-        rc = o.adaptive_sampling()
+        o.initial_sampling()
         t_mdas += time.perf_counter() - t0
-        if rc:
-            raise RuntimeError("oracle capacity")
-        o.eval_scene(params, o.n_samples - o.new_samples, o.new_samples)
-        passes += 1
-    t0 = time.perf_counter()
-    o.integrate(w["W"], w["H"])
-    t_mdas += time.perf_counter() - t0
-    n = o.n_samples
-    o.close()
-    log(f"[bench] cpu baseline: {n} samples in {t_mdas:.2f} s MDAS time, {passes} passes, "
-        f"{threads} threads")
-    return dict(value=round(n / t_mdas / 1e6, 3), unit="Msamples/s", cores=threads, kind="port",
+        o.eval_scene(params, 0, o.n_samples)
+        passes = 0
+        while o.n_samples < target:
+            t0 = time.perf_counter()
+            o.compute_errors()
+            rc = o.adaptive_sampling()
+            t_mdas += time.perf_counter() - t0
+            if rc:
+                raise RuntimeError("oracle capacity")
+            o.eval_scene(params, o.n_samples - o.new_samples, o.new_samples)
+            passes += 1
+        t0 = time.perf_counter()
+        o.integrate(w["W"], w["H"])
+        t_mdas += time.perf_counter() - t0
+        n = o.n_samples
+        o.close()
+        times.append(t_mdas)
+        log(f"[bench] cpu baseline run {r + 1}/{runs}: {n} samples in {t_mdas:.2f} s MDAS time, "
+            f"{passes} passes, {threads} threads")
+    t_med = sorted(times)[len(times) // 2]
+    return dict(value=round(n / t_med / 1e6, 3), unit="Msamples/s", cores=threads, kind="port",
                 sample=(f"full workload {name}" if shrink == 0 else
                         f"{w['W']}x{w['H']} version of {name} (extraImgBits {w['e']})")
-                + f", {n} samples, {passes} passes, MDAS phases only, one run")
+                + f", {n} samples, {passes} passes, MDAS phases only, median of {runs} runs "
+                  f"({', '.join(f'{t:.2f}' for t in times)} s)")
 
 
 def main():
