@@ -44,10 +44,14 @@ __host__ __device__ inline float bf2f(unsigned short h) {
 __host__ __device__ inline float bfround(float f) { return bf2f(f2bf(f)); }
 
 // GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))), with
-// tanh(z) = 1 - 2 / (exp(2z) + 1) (exact limits +-1 for large |z|).
+// tanh(z) = 1 - 2 / (exp(2z) + 1) (exact limits +-1 for large |z|).  The
+// quotient uses the hardware reciprocal (about 1 ulp) instead of the
+// correctly rounded division the build asks for elsewhere: the network's
+// activations are rounded to bf16 right after, and the division was the
+// largest part of the activation's cost.
 __device__ __forceinline__ float gelu_tanh(float x) {
     const float z = 0.7978845608f * (x + 0.044715f * x * x * x);
-    return 1.0f - 2.0f / (__expf(2.0f * z) + 1.0f);
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(__expf(2.0f * z) + 1.0f);
 }
 __device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + gelu_tanh(x)); }
 __device__ __forceinline__ float gelu_grad(float x) {
