@@ -118,6 +118,7 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
     __shared__ unsigned short actb[4][16 * kMlpH];   // layer-1 activations (bf16)
     __shared__ float ys[4][16];
     __shared__ __align__(16) unsigned short w1s[64 * 32], w2s[64 * 64];  // the bf16 weights
+    __shared__ float bl[3 * 64 + 1];  // b1, b2, bf16-rounded w3, b3
     if (pass_stopped(a.cnt)) return;
     const bool first = a.cnt->passes == 0;
     const int n_items = first ? a.cnt->n_leaves : a.cnt->new_nodes;
@@ -131,6 +132,12 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
             ((u16x8*)w1s)[k] = ((const u16x8*)l.mlp.w1p)[k];
         for (int k = threadIdx.x; k < 64 * 64 / 8; k += 256)
             ((u16x8*)w2s)[k] = ((const u16x8*)l.mlp.w2p)[k];
+        if (threadIdx.x < 64) {
+            bl[threadIdx.x] = l.mlp.p[kOffB1 + threadIdx.x];
+            bl[64 + threadIdx.x] = l.mlp.p[kOffB2 + threadIdx.x];
+            bl[128 + threadIdx.x] = l.mlp.w3r[threadIdx.x];
+        }
+        if (threadIdx.x == 0) bl[192] = l.mlp.p[kOffB3];
         __syncthreads();
     }
     float m = 0.0f;
@@ -240,7 +247,7 @@ __device__ __forceinline__ void errors_learned_body(const DevGrid& a, const Lear
         if (model) {
             for (int tile = 0; tile < 4 * hmax; ++tile) {
                 float y[4];
-                mlp16_bf(l.mlp, w1s, w2s, &xsb[w][16 * tile * kMlpIn], actb[w], y);
+                mlp16_bf(w1s, w2s, bl, &xsb[w][16 * tile * kMlpIn], actb[w], y);
                 if ((lane & 15) == 0) {
 #pragma unroll
                     for (int i = 0; i < 4; ++i) ys[w][4 * (lane >> 4) + i] = y[i];

@@ -172,9 +172,11 @@ __device__ __forceinline__ void mlp16(const MlpDev& M, const float* x, float* ac
 // the bf16 layer-1 activations kept as bf16 bits (act: LDS [16][64]): the
 // values are those of mlp16 (its rows and activations are rounded to bf16
 // before use), in half the LDS; the bf16 weights are read from the
-// workgroup's LDS copies w1s [64][32] and w2s [64][64].  Inference only.
-__device__ __forceinline__ void mlp16_bf(const MlpDev& M, const unsigned short* w1s,
-                                         const unsigned short* w2s, const unsigned short* x,
+// workgroup's LDS copies w1s [64][32] and w2s [64][64], and b1, b2, the
+// bf16-rounded w3 and b3 from its LDS copy bl [64 + 64 + 64 + 1].
+// Inference only.
+__device__ __forceinline__ void mlp16_bf(const unsigned short* w1s, const unsigned short* w2s,
+                                         const float* bl, const unsigned short* x,
                                          unsigned short* act, float y[4]) {
     const int lane = threadIdx.x & 63;
     const int col = lane & 15, kg = lane >> 4;
@@ -196,7 +198,7 @@ __device__ __forceinline__ void mlp16_bf(const MlpDev& M, const unsigned short* 
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int n = 16 * t + col;
-        const float bias = M.p[kOffB1 + n];
+        const float bias = bl[n];
 #pragma unroll
         for (int i = 0; i < 4; ++i) act[(4 * kg + i) * kMlpH + n] = f2bf(gelu(z[t][i] + bias));
     }
@@ -219,7 +221,7 @@ __device__ __forceinline__ void mlp16_bf(const MlpDev& M, const unsigned short* 
             c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[kc], b, c, 0, 0, 0);
         }
         const int n = 16 * t + col;
-        const float bias = M.p[kOffB2 + n], w3 = M.w3r[n];
+        const float bias = bl[64 + n], w3 = bl[128 + n];
 #pragma unroll
         for (int i = 0; i < 4; ++i) part[i] += bfround(gelu(c[i] + bias)) * w3;
     }
@@ -228,7 +230,7 @@ __device__ __forceinline__ void mlp16_bf(const MlpDev& M, const unsigned short* 
         float s = part[i];
 #pragma unroll
         for (int o = 1; o < 16; o <<= 1) s += __shfl_xor(s, o, 64);
-        y[i] = s + M.p[kOffB3];
+        y[i] = s + bl[192];
     }
     wave_sync_lds();
 }
