@@ -84,3 +84,37 @@ def test_random_config_bit_exact(mdas, c):
     g.close()
     o.close()
     sc.close()
+
+
+# grids that start from a single leaf (bitsPerDim 0, extraImgBits 0) or
+# reach their target with no pass at all, on 1- and 2-pixel sample spaces:
+# most of the launch's chunks and workgroups have no leaf to work on
+# dim, b, e, W, H, spp, scene, alpha, candidatesNum
+TINY = [(3, 0, 0, 1, 1, 8, 5, 0.25, 4), (4, 0, 0, 1, 1, 100, 5, 0.25, 4),
+        (5, 0, 0, 1, 1, 64, 0, 0.0, 1), (6, 0, 0, 2, 1, 50, 4, 1.0, 8),
+        (5, 1, 0, 1, 1, 40, 3, 0.25, 4), (3, 0, 1, 2, 2, 30, 1, 0.5, 2)]
+
+
+@pytest.mark.parametrize("c", TINY, ids=[f"tiny{i}-d{c[0]}" for i, c in enumerate(TINY)])
+def test_tiny_grid_bit_exact(mdas, c):
+    dim, b, e, W, H, spp, kind, alpha, cand = c
+    print(c, flush=True)
+    target = spp * W * H
+    cap = int(target * 1.6) + (4 << (b * dim + 2 * e)) + 64
+    args = (dim, cap, cand, b, e, alpha, float(W), float(H))
+    sc = mdas.Scene(kind, 5, dim, float(W), float(H))
+    g = mdas.KDGrid(*args)
+    o = pyoracle.OracleGrid(*args)
+    assert g.Run(sc, target) == o.run(sc.params_bytes(), target)
+    gs, os_ = g.download_state(), o.state()
+    for k in gs:
+        a, r = gs[k], os_[k]
+        eq = a.view(np.uint32) == r.view(np.uint32) if a.dtype == np.float32 else a == r
+        assert eq.all(), (k, np.argwhere(~eq)[:3].tolist())
+    for w, h in ((W, H), (3, 2)):
+        px = g.integrate_image(w, h)
+        assert np.array_equal(px.view(np.uint32), o.integrate(w, h).view(np.uint32))
+    assert g.Validate()
+    g.close()
+    o.close()
+    sc.close()
