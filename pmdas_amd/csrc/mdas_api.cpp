@@ -687,7 +687,7 @@ int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream)
     mdas::kbeg(g, st, "k_eval");
     HIP_TRY(mdas::launch_eval(g, s->dev, 0, 4 * L0, false, st));
     mdas::kend(g, st);
-    int enqueued = 0;
+    long long last_n = -1;  // samples after the last block of passes
     struct LoopMode {  // the pass kernels of this loop skip the stale markers
         mdas::DevGrid& d;
         explicit LoopMode(mdas::DevGrid& x) : d(x) { d.in_loop = 1; }
@@ -711,14 +711,16 @@ int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream)
             mdas::kbeg(g, st, "k_eval");
             HIP_TRY(mdas::launch_eval(g, s->dev, 0, 0, true, st));
             mdas::kend(g, st);
-            ++enqueued;
         }
         int rc = sync_counters(g, st);
         if (rc) return rc;
         if (g.host.done || g.host.status) break;
-        // every pass adds at least the sample of the max-error leaf, but a
-        // pathological integrand could make that one sample per pass
-        if (enqueued >= 20000) return fail(MDAS_ERR_STATE, "sampling loop exceeded 20000 passes");
+        // every pass selects the leaf of maximum error (its ratio is 1), so
+        // each block adds samples and the loop ends at the limit; a block that
+        // adds none means the errors are not finite (e.g. a NaN integrand)
+        if (g.host.n_samples == last_n)
+            return fail(MDAS_ERR_STATE, "a block of passes added no sample (non-finite errors)");
+        last_n = g.host.n_samples;
     }
     mdas::kbeg(g, st, "k_mark_last");
     HIP_TRY(mdas::launch_mark_last_children(g, st));

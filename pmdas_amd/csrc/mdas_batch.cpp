@@ -23,7 +23,7 @@ struct Group {
     mdas::DevGrid* d_grids = nullptr;    // device copies of the descriptors
     const void** d_params = nullptr;     // device array of scene parameter blocks
     int* d_limits = nullptr;
-    int* d_status = nullptr;             // (done, status) per member
+    int* d_status = nullptr;             // (done, status, samples) per member
     mdas::LearnDev* d_learn = nullptr;   // per member, when any member is in learned mode
     int blocks = 0;
     hipGraph_t graph = nullptr;
@@ -40,8 +40,9 @@ struct Group {
 __global__ void k_batch_status(const mdas::DevGrid* grids, int n, int* out) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) {
-        out[2 * t] = grids[t].cnt->done;
-        out[2 * t + 1] = grids[t].cnt->status;
+        out[3 * t] = grids[t].cnt->done;
+        out[3 * t + 1] = grids[t].cnt->status;
+        out[3 * t + 2] = grids[t].cnt->n_samples;
     }
 }
 
@@ -96,7 +97,7 @@ int mdas_batch_create(mdas_handle* grids, mdas_scene_handle* scenes, int n, mdas
         hipError_t e = hipMalloc(&gr.d_grids, sizeof(mdas::DevGrid) * m);
         if (e == hipSuccess) e = hipMalloc((void**)&gr.d_params, sizeof(void*) * m);
         if (e == hipSuccess) e = hipMalloc(&gr.d_limits, sizeof(int) * m);
-        if (e == hipSuccess) e = hipMalloc(&gr.d_status, sizeof(int) * 2 * m);
+        if (e == hipSuccess) e = hipMalloc(&gr.d_status, sizeof(int) * 3 * m);
         if (e == hipSuccess) e = hipMemcpy(gr.d_grids, dg.data(), sizeof(mdas::DevGrid) * m, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(gr.d_params, dp.data(), sizeof(void*) * m, hipMemcpyHostToDevice);
         if (e == hipSuccess && learned) e = hipMalloc(&gr.d_learn, sizeof(mdas::LearnDev) * m);
@@ -161,6 +162,7 @@ int mdas_batch_run(mdas_batch_handle b, const long long* targets, void* stream, 
         for (int k = 0; k < m; ++k) b->grids[g.members[k]]->g.initialized = true;
     }
     std::vector<char> finished(b->groups.size(), 0);
+    std::vector<int> last_n(n, -1);  // samples of each image after the last block
     int blocks_run = 0;
     for (;;) {
         for (size_t gi = 0; gi < b->groups.size(); ++gi) {
@@ -195,22 +197,29 @@ int mdas_batch_run(mdas_batch_handle b, const long long* targets, void* stream, 
             if (finished[gi]) continue;
             Group& g = b->groups[gi];
             const int m = (int)g.members.size();
-            std::vector<int> s(2 * m);
-            B_TRY(hipMemcpy(s.data(), g.d_status, sizeof(int) * 2 * m, hipMemcpyDeviceToHost));
+            std::vector<int> s(3 * m);
+            B_TRY(hipMemcpy(s.data(), g.d_status, sizeof(int) * 3 * m, hipMemcpyDeviceToHost));
             bool done = true;
             for (int k = 0; k < m; ++k) {
-                if (s[2 * k + 1] != 0) {
+                if (s[3 * k + 1] != 0) {
                     finished[gi] = 1;
-                    return mdas::set_error(s[2 * k + 1], "image " + std::to_string(g.members[k]) +
+                    return mdas::set_error(s[3 * k + 1], "image " + std::to_string(g.members[k]) +
                                                              ": adaptive pass would exceed max_samples");
                 }
-                done = done && s[2 * k] != 0;
+                // every pass selects the leaf of maximum error, so a block of
+                // passes that adds no sample means the errors are not finite
+                int& prev = last_n[g.members[k]];
+                if (s[3 * k] == 0 && s[3 * k + 2] == prev)
+                    return mdas::set_error(MDAS_ERR_STATE, "image " + std::to_string(g.members[k]) +
+                                                               ": a block of passes added no sample "
+                                                               "(non-finite errors)");
+                prev = s[3 * k + 2];
+                done = done && s[3 * k] != 0;
             }
             finished[gi] = done ? 1 : 0;
             all = all && done;
         }
         if (all) break;
-        if (blocks_run * B >= 20000) return mdas::set_error(MDAS_ERR_STATE, "batch exceeded 20000 passes");
     }
     for (Group& g : b->groups)
         B_TRY(mdas::launch_mark_last_children_b(g.d_grids, (int)g.members.size(), st));

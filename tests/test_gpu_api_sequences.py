@@ -133,3 +133,29 @@ def test_repeated_runs(mdas):
     fresh.close()
     g.close()
     sc.close()
+
+
+def test_run_of_many_passes(mdas):
+    """A run that needs 21287 passes (alpha 1 on the disk scene from a single
+    initial leaf selects a handful of leaves per pass) goes to its target
+    like the oracle's loop, which has no pass limit; the same image in a
+    batch too (every pass adds the sample of the max-error leaf, so the run
+    always ends)."""
+    target = 200000
+    args = (3, int(target * 1.6) + 68, 4, 0, 0, 1.0, 1.0, 1.0)
+    sc = mdas.Scene(1, 3, 3, 1.0, 1.0)
+    g = mdas.KDGrid(*args)
+    o = pyoracle.OracleGrid(*args)
+    passes = g.Run(sc, target)
+    assert passes == o.run(sc.params_bytes(), target) == 21287
+    _same(g.download_state(), o.state(), "many passes")
+    ref = g.download_state()
+    g2 = mdas.KDGrid(*args)
+    b = mdas.Batch([g2], [sc])
+    b.run([target])
+    _same(g2.download_state(), ref, "many passes, batch")
+    b.close()
+    g2.close()
+    g.close()
+    o.close()
+    sc.close()

@@ -3,9 +3,10 @@ bitsPerDim 0..2, extraImgBits, alpha (including 0 and 1), candidatesNum 1..8,
 every analytic scene of the dimension, odd image sizes and sample targets.
 Each runs the device-side loop (optixMotionBlur.cpp:768-791) and the
 reconstruction, and must equal the oracle bit for bit: final state, pass
-count, image and sRGB bytes.  MDAS_RANDOM_CONFIGS and MDAS_RANDOM_SEED
-(environment) set the matrix for a one-off sweep; the default is 200
-configurations from seed 2024 (about 5 s on one MI355X)."""
+count, image and sRGB bytes.  MDAS_RANDOM_CONFIGS, MDAS_RANDOM_SEED and
+MDAS_RANDOM_SCALE (image sides multiplied by it) set the matrix for a
+one-off sweep; the default is 200 configurations from seed 2024 at scale 1
+(about 5 s on one MI355X)."""
 import os
 import random
 
@@ -19,7 +20,7 @@ pytestmark = pytest.mark.gpu
 SCENES = {3: (1, 5, 0), 4: (2, 5, 0), 5: (3, 5, 0), 6: (4, 0)}
 
 
-def _configs(n=200, seed=2024):
+def _configs(n=200, seed=2024, scale=1):
     rng = random.Random(seed)
     out = []
     for k in range(n):
@@ -28,7 +29,7 @@ def _configs(n=200, seed=2024):
         e = rng.randint(1, 4)
         while b * dim + 2 * e > 12:
             e -= 1
-        W, H = rng.randint(5, 120), rng.randint(3, 90)
+        W, H = rng.randint(5, 120) * scale, rng.randint(3, 90) * scale
         spp = rng.choice((4, 6, 9, 16, 24))
         out.append(dict(dim=dim, b=b, e=e, W=W, H=H, spp=spp,
                         alpha=rng.choice((0.0, 0.125, 0.25, 0.5, 1.0)),
@@ -38,7 +39,8 @@ def _configs(n=200, seed=2024):
 
 
 CONFIGS = _configs(int(os.environ.get("MDAS_RANDOM_CONFIGS", "200")),
-                   int(os.environ.get("MDAS_RANDOM_SEED", "2024")))
+                   int(os.environ.get("MDAS_RANDOM_SEED", "2024")),
+                   int(os.environ.get("MDAS_RANDOM_SCALE", "1")))
 
 
 @pytest.fixture(scope="module")
