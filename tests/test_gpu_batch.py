@@ -141,3 +141,31 @@ def test_batch_autotune_keeps_results(kd):
             assert np.array_equal(st[k], ref[k]), k
         solo.close()
     batch.close()
+
+
+def test_random_batch_matches_oracle(kd):
+    """24 seeded random small images of every dimension and scene (the random
+    parity matrix's generator, another seed) in one batch: each equals the
+    oracle's run of it alone, bit for bit."""
+    from test_gpu_random_configs import _configs
+    made = []
+    for c in _configs(24, 77):
+        W, H, dim = c["W"], c["H"], c["dim"]
+        target = max(1, int(c["spp"] * W * H))
+        cap = int(target * 1.6) + (4 << (c["b"] * dim + 2 * c["e"])) + 64
+        args = (dim, cap, c["cand"], c["b"], c["e"], c["alpha"], float(W), float(H))
+        made.append((kd.KDGrid(*args), kd.Scene(c["kind"], c["seed"], dim, float(W), float(H)),
+                     target, args))
+    batch = kd.Batch([m[0] for m in made], [m[1] for m in made])
+    batch.run([m[2] for m in made], use_graph=True)
+    for g, sc, target, args in made:
+        o = pyoracle.OracleGrid(*args)
+        o.run(sc.params_bytes(), target)
+        st, ref = g.download_state(), o.state()
+        for k in st:
+            assert np.array_equal(st[k], ref[k]), (k, args)
+        o.close()
+    batch.close()
+    for g, sc, _, _ in made:
+        g.close()
+        sc.close()
