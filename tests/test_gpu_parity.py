@@ -314,3 +314,35 @@ def test_log_keys_per_pass(mdas, tmp_path):
     times = sum((_log_values(log, k) for k in ("COMPUTE ERRORS TIME", "ADAPTIVE SAMPLING TIME",
                                                 "TRACE TIME", "INTEGRATE TIME")), [])
     assert all(t >= 0 for t in times) and sum(times) > 0
+
+
+def _random_runs():
+    from test_gpu_random_configs import _configs
+    return _configs(16, 31)
+
+
+@pytest.mark.parametrize("c", _random_runs(), ids=lambda c: f"rnd{c['id']}-d{c['dim']}")
+def test_pass_by_pass_random_configs(mdas, c):
+    """The one-call-at-a-time sequence (ComputeErrors, AdaptiveSampling, the
+    integrand on the new slice) over seeded random configurations, against
+    the oracle after every pass, for up to 60 passes."""
+    g, o, sc, target = _make(mdas, c["kind"], c["dim"], c["b"], c["e"], c["W"], c["H"],
+                             c["spp"], c["alpha"], c["cand"], seed=c["seed"])
+    print(c, flush=True)
+    params = sc.params_bytes()
+    g.InitialSampling()
+    g.EvalIntegrand(sc, 0, g.GetNumberOfSamples())
+    o.initial_sampling()
+    o.eval_scene(params, 0, o.n_samples)
+    p = 0
+    while g.GetNumberOfSamples() < target and p < 60:
+        g.ComputeErrors()
+        o.compute_errors()
+        assert np.float32(g.GetMaxError()) == np.float32(o.max_error)
+        g.AdaptiveSampling()
+        assert o.adaptive_sampling() == 0
+        g.EvalNewSamples(sc)
+        o.eval_scene(params, o.n_samples - o.new_samples, o.new_samples)
+        _same_state(g.download_state(), o.state(), f"pass {p}")
+        p += 1
+    assert g.Validate()
