@@ -136,8 +136,9 @@ def batch_images(n=32, seed=7):
 
 def run_batch(steps, log=print):
     """The batch workload: 32 images of uneven resolution, 5D, 32 spp, advanced as one
-    batch replayed from a captured hipGraph, then reconstructed one by one
-    with the gather tile the autotuner picked for each image in the warmup."""
+    batch replayed from a captured hipGraph, then reconstructed by a second
+    captured graph holding the 32 reconstructions, each with the gather tile
+    the autotuner picked for its image in the warmup."""
     from pmdas_amd import hip, kdgrid
     imgs = batch_images()
     grids, scenes, targets, bufs = [], [], [], []
@@ -152,14 +153,15 @@ def run_batch(steps, log=print):
     shape = batch.autotune(targets)
     log(f"[bench] batch of {len(imgs)}: autotuned launch shape {shape} workgroups")
     times = []
+    Ws, Hs = [w["W"] for w in imgs], [w["H"] for w in imgs]
     for i in range(steps + 1):
         hip.synchronize()
         t0 = time.perf_counter()
         batch.run(targets, use_graph=True)
-        for g, w, b in zip(grids, imgs, bufs):
-            if i == 0:  # warmup: the autotuner picks each image's gather tile
+        if i == 0:  # warmup: the autotuner picks each image's gather tile
+            for g, w, b in zip(grids, imgs, bufs):
                 g.autotune_recon(b.ptr, w["W"], w["H"])
-            g.Integrate(b.ptr, w["W"], w["H"])
+        batch.integrate(bufs, Ws, Hs, use_graph=True)
         hip.synchronize()
         t1 = time.perf_counter()
         if i > 0:
@@ -171,7 +173,8 @@ def run_batch(steps, log=print):
     res = dict(images=len(imgs), pixels=sum(w["W"] * w["H"] for w in imgs), samples=n,
                ms_per_step=round(ms, 2), e2e_msamples_per_s=round(n / (ms * 1e3), 1),
                passes_max=max(g.pass_stats().shape[0] for g in grids), launch_shape=shape,
-               note="wall time of batch run (with the integrand) + 32 reconstructions")
+               note="wall time of the batch run (with the integrand) and the 32 "
+                    "reconstructions, both replayed from captured hipGraphs")
     batch.close()
     for b in bufs:
         b.free()
@@ -356,6 +359,58 @@ def phase_model_bytes(w, pstats, L, N, scatter):
     return b
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def device_record():
+    """What can tell two GPU machines apart when the same build times
+    differently: product name, compute units and peak engine clock
+    (rocminfo), current clocks and compute / memory partition modes
+    (rocm-smi).  Both run as child processes before this one touches the
+    GPU; a tool that is missing or fails leaves its fields out."""
+    import subprocess
+    rec = {}
+    try:
+        r = subprocess.run(["rocminfo"], capture_output=True, text=True, timeout=60)
+        agent, gpu = {}, None
+        for ln in r.stdout.splitlines():
+            ln = ln.strip()
+            if ln.startswith("Agent "):
+                if agent.get("Device Type") == "GPU" and gpu is None:
+                    gpu = agent
+                agent = {}
+            elif ":" in ln:
+                k, v = ln.split(":", 1)
+                agent.setdefault(k.strip(), v.strip())
+        if agent.get("Device Type") == "GPU" and gpu is None:
+            gpu = agent
+        if gpu:
+            for k in ("Marketing Name", "Name", "Compute Unit", "Max Clock Freq. (MHz)",
+                      "SIMDs per CU", "Shader Engines"):
+                if k in gpu:
+                    rec[k] = gpu[k]
+    except Exception as e:  # noqa: BLE001 - a record, not a requirement
+        rec["rocminfo_error"] = str(e)[:120]
+    try:
+        r = subprocess.run(["rocm-smi", "--showproductname", "--showclocks", "--showcomputepartition",
+                            "--showmemorypartition", "--json"], capture_output=True, text=True,
+                           timeout=60)
+        js = json.loads(r.stdout[r.stdout.index("{"):])
+        card = next((v for k, v in sorted(js.items()) if k.startswith("card")), {})
+        rec["rocm_smi"] = {k: v for k, v in card.items() if isinstance(v, str) and len(v) < 80}
+    except Exception as e:  # noqa: BLE001
+        rec["rocm_smi_error"] = str(e)[:120]
+    return rec
+
+
 def copy_probe():
     """Achievable HBM bandwidth of this GPU: best of 5 plain float4 copies of
     1 GiB (pmdas_amd/bw_probe, read + write bytes / time), run as a child
@@ -419,8 +474,12 @@ def cpu_baseline(name, shrink=0, log=print, runs=3):
     w["e"] -= shrink
     target, cap = capacity(w)
     lib = pyoracle.load()
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-    threads = min(threads, 16)
+    # threads: the CPU share this process is given for its one GPU
+    # (OMP_NUM_THREADS, which the GPU machines set to 16 per GPU), else every
+    # core it may run on
+    visible = len(os.sched_getaffinity(0))
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or visible
+    threads = max(1, min(threads, visible))
     lib.orc_set_threads(threads)
     params = pyoracle.make_scene(w["kind"], w["seed"], w["dim"], float(w["W"]), float(w["H"]))
     times = []
@@ -452,6 +511,8 @@ def cpu_baseline(name, shrink=0, log=print, runs=3):
             f"{passes} passes, {threads} threads")
     t_med = sorted(times)[len(times) // 2]
     return dict(value=round(n / t_med / 1e6, 3), unit="Msamples/s", cores=threads, kind="port",
+                cpu=cpu_model(), cores_visible=visible,
+                threads_from="OMP_NUM_THREADS" if os.environ.get("OMP_NUM_THREADS") else "affinity",
                 sample=(f"full workload {name}" if shrink == 0 else
                         f"{w['W']}x{w['H']} version of {name} (extraImgBits {w['e']})")
                 + f", {n} samples, {passes} passes, MDAS phases only, median of {runs} runs "
@@ -477,7 +538,9 @@ def main():
               "sharding by image or screen tile is a design, not built)", file=sys.stderr)
         sys.exit(2)
     log = lambda m: print(m, file=sys.stderr, flush=True)
-    copy_gbs = copy_probe()  # before this process initialises the GPU
+    dev_rec = device_record()  # child processes, before this one initialises the GPU
+    log(f"[bench] device: {dev_rec}")
+    copy_gbs = copy_probe()
     log(f"[bench] copy probe: {copy_gbs} GB/s")
     import pmdas_amd
     from pmdas_amd import hip
@@ -519,7 +582,8 @@ def main():
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 4),
                         "GBps": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)}
                     for k, v in sorted(res["kernels"].items(), key=lambda kv: -kv[1]["ms"])},
-        "device": hip.device_name(),
+        "device": dev_rec.get("Marketing Name") or hip.device_name() or "unknown",
+        "device_record": dev_rec,
     }
     if not a.no_secondary:
         other = [k for k in WORKLOADS if k != a.workload][0]

@@ -61,6 +61,15 @@ int mdas_create(const mdas_config* cfg, mdas_handle* out);
 /* KDGrid::~KDGrid (buffers freed; buffer.h:27-29) */
 int mdas_destroy(mdas_handle h);
 
+/* The constructor's std::ofstream* log (kdgrid.h:50, kdgrid.cu:609-630) for C
+ * callers: fn(user, key, value, is_count) receives every "KEY", value pair the
+ * reference writes to that stream (kdgrid.cu:670-671, 713, 769-770, 816-817;
+ * is_count = 1 for sample counts, 0 for times in ms), in the same order, in
+ * addition to the config's log_path file.  fn = NULL turns it off.
+ * include/mdas_kdgrid.hpp uses it to write to the app's std::ofstream. */
+typedef void (*mdas_log_fn)(void* user, const char* key, double value, int is_count);
+int mdas_set_log_sink(mdas_handle h, mdas_log_fn fn, void* user);
+
 /* KDGrid::InitialSampling (kdgrid.h:53; kdgrid.cu:633-674) */
 int mdas_initial_sampling(mdas_handle h, void* stream);
 /* KDGrid::ComputeErrors (kdgrid.h:55; kdgrid.cu:676-716) */
@@ -71,7 +80,11 @@ int mdas_adaptive_sampling(mdas_handle h, void* stream);
 /* KDGrid::SamplingPass (kdgrid.h:58; kdgrid.cu:775-779) */
 int mdas_sampling_pass(mdas_handle h, void* stream);
 /* KDGrid::Integrate (kdgrid.h:59; kdgrid.cu:781-820).  `pixels` is a device
- * float4[width*height]; every pixel is written (xyz = estimate, w = 0). */
+ * float4[width*height]; every pixel is written (xyz = estimate, w = 0).
+ * Stream-ordered, like the reference: the buffers are sized from the leaf
+ * count the host already holds (a call allocates only when they grow), except
+ * for trees whose key bound exceeds 8 GiB (initial cells spanning many
+ * pixel tiles), where the key count is read back first. */
 int mdas_integrate(mdas_handle h, float* pixels, int width, int height, void* stream);
 /* Reference-style reconstruction with float atomics (kdgrid.cu:483-582);
  * same result up to summation order.  Kept for comparison. */
@@ -130,8 +143,13 @@ int mdas_render_qmc(mdas_scene_handle s, int width, int height, int spp, int fir
 
 /* Whole sampling loop without per-pass host round trips: initial sampling,
  * integrand, then passes while N < min(target_samples, max_samples)
- * (optixMotionBlur.cpp:768-791).  The stopping rule is evaluated on the
- * device; the host checks it once every `passes_per_sync` passes. */
+ * (optixMotionBlur.cpp:768-791).  With a target at or beyond max_samples the
+ * loop also keeps N + L <= max_samples, the capacity guard of the app's
+ * interactive loop (optixMotionBlur.cpp:723-724), so such a run ends cleanly
+ * (MDAS_OK) short of the capacity instead of overflowing; with a smaller
+ * target a pass that would overflow returns MDAS_ERR_CAPACITY.  The stopping
+ * rule is evaluated on the device; the host checks it once every few
+ * passes. */
 int mdas_run(mdas_handle h, mdas_scene_handle s, long long target_samples, void* stream);
 int mdas_get_passes(mdas_handle h);
 
@@ -198,6 +216,15 @@ int mdas_batch_create(mdas_handle* grids, mdas_scene_handle* scenes, int n, mdas
 int mdas_batch_destroy(mdas_batch_handle b);
 /* targets[i]: sample target of image i; use_graph: replay a captured graph */
 int mdas_batch_run(mdas_batch_handle b, const long long* targets, void* stream, int use_graph);
+/* Reconstruction of every image i (kdgrid.cu:781-820) into its device float4
+ * image pixels[i] of widths[i] x heights[i], each with its grid's gather
+ * tile; the same pixels as mdas_integrate on each grid.  With use_graph the
+ * reconstructions of all images are one captured hipGraph, replayed while the
+ * images' sizes, tiles, outputs and buffers stay the same (the buffers are
+ * sized before the capture, from each image's leaf count).  Stream-ordered;
+ * the caller synchronises. */
+int mdas_batch_integrate(mdas_batch_handle b, float* const* pixels, const int* widths,
+                         const int* heights, void* stream, int use_graph);
 /* wall time of the last run (ms), blocks of passes run, dimension groups */
 int mdas_batch_info(mdas_batch_handle b, double* wall_ms, int* pass_blocks, int* groups);
 /* Runs the batch (with graph replay) once per candidate launch shape --

@@ -5,14 +5,22 @@
 // The reference apps use KDGrid through these calls only
 // (SDK/optixMotionBlur/optixMotionBlur.cpp:331-360, 473-491, 768-806):
 //   new KDGrid<Point>(maxSamples, candidatesNum, bitsPerDim, extraImgBits,
-//                     alpha, scaleX, scaleY, &log);
-//   InitialSampling(); SamplingPass(); Integrate(px, w, h);
-//   ConvertToBytes(px, bytes, w, h); SamplingDensity(hostPx, w, h);
-//   Validate(); GetSampleCoordinates().Data(); GetSampleValues().Data();
-//   GetNewSamples(); GetNumberOfSamples(); GetNumberOfNodes(); GetScaleX/Y().
-// Those compile unchanged against this header.  Failures throw
-// mdas::Error (the reference printed CUDA errors and continued).
+//                     alpha, scaleX, scaleY, &log);       // std::ofstream*
+//   InitialSampling(); SamplingPass(); Validate();
+//   Integrate(float4*, w, h); ConvertToBytes(float4*, uchar4*, w, h);
+//   SamplingDensity(float4* host, w, h);
+//   reinterpret_cast<float*>(GetSampleCoordinates().Data());
+//   float3* values = GetSampleValues().Data();
+//   GetNewSamples(); GetNewNodes(); GetNumberOfSamples(); GetNumberOfNodes();
+//   GetScaleX(); GetScaleY().
+// Those lines compile unchanged against this header with HIP's vector types
+// (float2/3/4, uchar4; hip/hip_vector_types.h, included here), and the log
+// stream receives the reference's "KEY\nvalue" lines (kdgrid.cu:670-817).
+// Failures throw mdas::Error (the reference printed CUDA errors and
+// continued).
 #pragma once
+#include <hip/hip_vector_types.h>
+
 #include <fstream>
 #include <stdexcept>
 #include <string>
@@ -51,18 +59,26 @@ struct DeviceView {
     T* Data() { return ptr; }
 };
 
-// float3 layout of a sample value (SDK/cuda/whitted.h:80).
-struct Value3 {
-    float x, y, z;
-};
+static_assert(sizeof(float3) == 12, "a sample value is three packed floats (whitted.h:78)");
+
+// Writes the library's log lines to the app's stream as the reference does
+// (`*log << "KEY\n" << value << std::endl`): counts as integers, times as
+// floats.
+inline void ofstream_log_sink(void* user, const char* key, double value, int is_count) {
+    std::ofstream& os = *static_cast<std::ofstream*>(user);
+    if (is_count)
+        os << key << "\n" << (long long)value << std::endl;
+    else
+        os << key << "\n" << (float)value << std::endl;
+}
 
 template <typename Point>
 class KDGrid {
 public:
-    // kdgrid.h:42-51.  The log stream is replaced by a log file path: the
-    // library appends the reference's "KEY\nvalue" lines to it.
+    // kdgrid.h:42-51.  `stream` (an addition) is the HIP stream every call is
+    // enqueued on; the reference uses stream 0.
     KDGrid(int maxSamples, int candidatesNum, int bitsPerDim, int extraImgBits, float alpha,
-           float scaleX, float scaleY, const char* logPath = nullptr, void* stream = nullptr)
+           float scaleX, float scaleY, std::ofstream* log = nullptr, void* stream = nullptr)
         : stream_(stream) {
         mdas_config c;
         c.dim = Point::DIM;
@@ -73,8 +89,9 @@ public:
         c.alpha = alpha;
         c.scale_x = scaleX;
         c.scale_y = scaleY;
-        c.log_path = logPath;
+        c.log_path = nullptr;
         check(mdas_create(&c, &h_));
+        if (log) check(mdas_set_log_sink(h_, &ofstream_log_sink, log));
     }
     ~KDGrid() { mdas_destroy(h_); }
     KDGrid(const KDGrid&) = delete;
@@ -84,16 +101,17 @@ public:
     void ComputeErrors() { check(mdas_compute_errors(h_, stream_)); }
     void AdaptiveSampling() { check(mdas_adaptive_sampling(h_, stream_)); }
     void SamplingPass() { check(mdas_sampling_pass(h_, stream_)); }
-    // float4 device buffers (CUDAOutputBuffer<float4> in the reference apps)
-    void Integrate(void* pixels, int width, int height) {
-        check(mdas_integrate(h_, (float*)pixels, width, height, stream_));
+    // kdgrid.h:59-61: device float4 / uchar4 images, host float4 density
+    void Integrate(float4* pixels, int width, int height) {
+        check(mdas_integrate(h_, reinterpret_cast<float*>(pixels), width, height, stream_));
     }
-    void SamplingDensity(void* hostPixels, int width, int height) {
-        check(mdas_sampling_density(h_, (float*)hostPixels, width, height));
+    void SamplingDensity(float4* hostPixels, int width, int height) {
+        check(mdas_sampling_density(h_, reinterpret_cast<float*>(hostPixels), width, height));
     }
-    void ConvertToBytes(void* pixels, void* pixelsBytes, int width, int height) {
-        check(mdas_convert_to_bytes(h_, (const float*)pixels, (unsigned char*)pixelsBytes, width,
-                                    height, stream_));
+    void ConvertToBytes(float4* pixels, uchar4* pixelsBytes, int width, int height) {
+        check(mdas_convert_to_bytes(h_, reinterpret_cast<const float*>(pixels),
+                                    reinterpret_cast<unsigned char*>(pixelsBytes), width, height,
+                                    stream_));
     }
     bool Validate() {
         int v = 0;
@@ -101,8 +119,13 @@ public:
         return v != 0;
     }
 
-    DeviceView<Point> GetSampleCoordinates() { return {(Point*)mdas_get_sample_coordinates(h_)}; }
-    DeviceView<Value3> GetSampleValues() { return {(Value3*)mdas_get_sample_values(h_)}; }
+    // kdgrid.h:65-66: the sample buffers the integrand reads and writes
+    DeviceView<Point> GetSampleCoordinates() {
+        return {reinterpret_cast<Point*>(mdas_get_sample_coordinates(h_))};
+    }
+    DeviceView<float3> GetSampleValues() {
+        return {reinterpret_cast<float3*>(mdas_get_sample_values(h_))};
+    }
 
     int GetCandidatesNum() { return mdas_get_candidates_num(h_); }
     int GetMaxNodeSize() { return mdas_get_max_node_size(h_); }

@@ -496,6 +496,10 @@ inline void make_color(const float* c, unsigned char* out) {
     out[3] = 255;
 }
 
+// kdgrid.cu:848-961: the reference's four conditions -- every sample inside
+// its leaf's box, the boxes' volumes summing to scaleX * scaleY (1e-2
+// relative), as many slots filled as samples, every sample referenced once
+// -- plus a slot index below N (the reference would read past the arrays).
 template <int D>
 int validate(Grid& g) {
     bool ok = true;
@@ -505,38 +509,27 @@ int validate(Grid& g) {
     for (long long i = 0; i < g.L; ++i) {
         const float* mn = &g.box[i * 2 * D];
         const float* mx = mn + D;
-        int n = 0;
-        bool prefix = true;
         for (int s = 0; s < 4; ++s) {
             int k = g.idx[4 * i + s];
-            if (k >= 0) {
-                if (!prefix || k >= g.N) {
-                    ok = false;
-                    continue;
-                }
-                for (int d = 0; d < D; ++d) {
-                    float c = g.coords[(long long)k * D + d];
-                    if (c < mn[d] || c > mx[d]) ok = false;  // aabb.h:Contains
-                }
-                hist[k]++;
-                ++n;
-            } else {
-                if (k != ~D) ok = false;
-                prefix = false;
+            if (k < 0) continue;  // an empty slot (kdgrid.cu:877)
+            if (k >= g.N) {
+                ok = false;
+                continue;
             }
+            for (int d = 0; d < D; ++d) {
+                float c = g.coords[(long long)k * D + d];
+                if (c < mn[d] || c > mx[d]) ok = false;  // aabb.h:Contains
+            }
+            hist[k]++;
+            ++total;
         }
-        if (n < 2) ok = false;
-        for (int d = 0; d < D; ++d)
-            if (!(mn[d] < mx[d])) ok = false;
         totalVol += (double)volume<D>(mn, mx);
-        total += n;
     }
     double root = (double)g.sX * (double)g.sY;
     if (std::fabs(totalVol - root) > 1.0e-2 * root) ok = false;  // kdgrid.cu:909-914
     if (total != g.N) ok = false;                                 // :916-920
     for (long long k = 0; k < g.N; ++k)
         if (hist[k] != 1) ok = false;  // :922-956
-    if (2 * g.L > g.N) ok = false;
     return ok ? 1 : 0;
 }
 
@@ -746,7 +739,11 @@ int orc_run(orc_grid* h, const void* params, long long target, int* passes) {
     orc_eval_scene(params, g.coords.data(), g.D, 0, g.N, g.values.data());
     long long limit = std::min(target, g.cap_s);
     int p = 0;
-    while (g.N < limit) {
+    // optixMotionBlur.cpp:781 (N < min(target, max_samples)); when the limit
+    // is the capacity, also no pass that could overflow it: N + L <=
+    // max_samples, the guard of the app's interactive loop (:723-724)
+    const bool guard = target >= g.cap_s;
+    while (g.N < limit && (!guard || g.N + g.L <= g.cap_s)) {
         orc_compute_errors(h);
         rc = orc_adaptive_sampling(h);
         if (rc) return rc;

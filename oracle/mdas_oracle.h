@@ -40,8 +40,10 @@ void orc_convert_to_bytes(const float* pixels, unsigned char* out, long long n);
 void orc_sampling_density(orc_grid* g, float* pixels, int w, int h); /* kdgrid.cu:822-837 */
 int orc_validate(orc_grid* g);                    /* kdgrid.cu:848-961, always on */
 
-/* Whole run with the sample app's stopping rule (optixMotionBlur.cpp:768-791),
- * using the analytic scene `params` (mdas_scene::Params layout) as integrand. */
+/* Whole run with the sample app's stopping rule (optixMotionBlur.cpp:768-791)
+ * plus, for a target at or beyond max_samples, its interactive loop's capacity
+ * guard (no pass once N + L > max_samples, :723-724), using the analytic scene `params` (mdas_scene::Params layout) as
+ * integrand. */
 int orc_run(orc_grid* g, const void* scene_params, long long target_samples, int* passes);
 /* Scene parameter block (same generator as the product's mdas_scene_create). */
 long long orc_make_scene(int kind, uint32_t seed, int dim, float scale_x, float scale_y, void* out,

@@ -129,6 +129,7 @@ SIGNATURES = {
     "mdas_run": (_I, [_H, _H, _LL, _P]),
     "mdas_get_passes": (_I, [_H]),
     "mdas_set_timing": (_I, [_H, _I]),
+    "mdas_set_log_sink": (_I, [_H, _P, _P]),
     "mdas_get_phase_times": (_I, [_H, ctypes.POINTER(MdasPhaseTimes)]),
     "mdas_get_pass_stats": (_I, [_H, _P, _I]),
     "mdas_kernel_stats": (_I, [_H, _I, ctypes.c_char_p, _I, ctypes.POINTER(_I),
@@ -143,6 +144,7 @@ SIGNATURES = {
     "mdas_batch_create": (_I, [_P, _P, _I, ctypes.POINTER(_H)]),
     "mdas_batch_destroy": (_I, [_H]),
     "mdas_batch_run": (_I, [_H, _P, _P, _I]),
+    "mdas_batch_integrate": (_I, [_H, _P, _P, _P, _P, _I]),
     "mdas_batch_info": (_I, [_H, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I),
                              ctypes.POINTER(_I)]),
     "mdas_batch_autotune": (_I, [_H, _P, _P, ctypes.POINTER(_I)]),

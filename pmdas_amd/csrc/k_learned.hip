@@ -319,7 +319,7 @@ __device__ __forceinline__ void err_reduce_learned_body(const DevGrid& a, const 
     __shared__ unsigned sm[16];
     __shared__ int ss[16], sv[16];
     if (a.cnt->done) return;
-    if (a.cnt->n_samples >= a.cnt->limit) {  // the loop stops before this pass
+    if (loop_limit_reached(a.cnt)) {  // the loop stops before this pass
         if (threadIdx.x == 0) a.cnt->done = 1;
         return;
     }

@@ -17,6 +17,8 @@
 //     reference's own expressions, including its accumulated pixel bounds
 //     (pMax += step), so a pixel is the sum of its contributions in leaf
 //     index order.
+#include <algorithm>
+#include <cmath>
 #include <type_traits>
 
 #include "mdas_leaf.hpp"
@@ -118,8 +120,10 @@ __global__ __launch_bounds__(256) void k_recon_prep(DevGrid a, ReconGeo G) {
     if (threadIdx.x == 0) a.part[blockIdx.x] = make_int4(red[0] + red[1] + red[2] + red[3], 0, 0, 0);
 }
 
-// Exclusive scan of part[].x into part_off[].x; total -> cnt->n_keys.
-__global__ __launch_bounds__(1024) void k_scan_keys(DevGrid a, int G) {
+// Exclusive scan of part[].x into part_off[].x; total -> cnt->n_keys.  A
+// total beyond the key buffer (cap_keys) is recorded in recon_overflow and
+// leaves n_keys = 0, so that no later kernel reads or writes past the buffer.
+__global__ __launch_bounds__(1024) void k_scan_keys(DevGrid a, int G, long long cap_keys) {
     __shared__ int ls[16];
     const int per = (G + 1023) / 1024;
     const int g0 = threadIdx.x * per;
@@ -147,14 +151,19 @@ __global__ __launch_bounds__(1024) void k_scan_keys(DevGrid a, int G) {
             a.part_off[g0 + k] = make_int2(r, 0);
             r += c;
         }
-    if (threadIdx.x == 0) a.cnt->n_keys = t;
+    if (threadIdx.x == 0) {
+        const bool over = (long long)t > cap_keys;
+        a.cnt->n_keys = over ? 0 : t;
+        a.cnt->recon_overflow = over ? t : 0;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_recon_emit(DevGrid a, ReconGeo G,
                                                     unsigned long long* keys, long long cap_keys) {
     __shared__ int lds[4];
     const int L = a.cnt->n_leaves;
-    if ((long long)a.cnt->n_keys > cap_keys) return;  // host grows the buffer first
+    if (a.cnt->recon_overflow != 0) return;  // the keys do not fit the buffer
+    (void)cap_keys;
     int beg, end;
     chunk_range(L, gridDim.x, blockIdx.x, beg, end);
     int base = a.part_off[blockIdx.x].x;
@@ -665,69 +674,110 @@ static ReconGeo make_geo(const Grid& g, int W, int H) {
     default: return hipErrorInvalidValue;              \
     }
 
-// Per-leaf records, tile counts and their scan, then (synchronising) make
-// sure the key buffers hold the keys.
-hipError_t recon_prepare(Grid& g, int W, int H, hipStream_t st) {
+static hipError_t grow_keys(Grid& g, long long cap) {
+    if (g.keys) (void)hipFree(g.keys);
+    if (g.keys_alt) (void)hipFree(g.keys_alt);
+    g.keys = g.keys_alt = g.keys_sorted = nullptr;
+    g.keys_cap = 0;
+    hipError_t e = hipMalloc(&g.keys, sizeof(unsigned long long) * (size_t)cap);
+    if (e == hipSuccess) e = hipMalloc(&g.keys_alt, sizeof(unsigned long long) * (size_t)cap);
+    if (e != hipSuccess) return e;
+    g.keys_cap = cap;
+    return hipSuccess;
+}
+
+// Upper bound on the keys one leaf emits: the tiles its pixel span can touch.
+// A leaf's box is at most g.ext_x by g.ext_y in x and y (splits only shrink
+// boxes), and a span of extent ext over n pixels of a sample space of size s
+// holds at most floor(ext n / s) + 2 pixels (floor(mx n/s) - floor(mn n/s) + 1,
+// with a margin for the rounding of the float products); p pixels starting
+// anywhere in a tile of T touch at most (p + T - 2) / T + 1 tiles.
+static long long tiles_per_leaf(const Grid& g, const ReconGeo& G) {
+    auto axis = [](double ext, int n, double s, int T, int tiles) {
+        long long p = (long long)std::floor(ext * (double)n / s * (1.0 + 1e-6) + 1e-3) + 2;
+        p = std::min<long long>(p, n);
+        return std::min<long long>((p + T - 2) / T + 1, tiles);
+    };
+    return axis(g.ext_x, G.W, G.sX, G.TW, G.tilesX) * axis(g.ext_y, G.H, G.sY, G.TH, G.tilesY);
+}
+
+// Key buffers of at most this many bytes are sized from the bound up front
+// (no host round trip in the reconstruction); larger bounds (tiny initial
+// grids, whose leaves span many tiles) size them from the counted keys.
+static const long long kKeyBoundBytes = 8ll << 30;
+
+// Sizes the reconstruction buffers for the current leaf count (g.host is
+// current after every call that changes the tree) and image; allocates only
+// when they grow.  Sets g.recon_bounded when the key buffers hold the bound,
+// i.e. the reconstruction needs no host round trip.
+hipError_t recon_reserve(Grid& g, int W, int H) {
     ReconGeo G = make_geo(g, W, H);
-    g.recon_pixels = (long long)W * H;
-    {
-        // records for the current leaves (the count is read back first)
-        int L = 0;
-        hipError_t e0 = hipMemcpyAsync(&L, &g.dev.cnt->n_leaves, sizeof(int), hipMemcpyDeviceToHost, st);
-        if (e0 == hipSuccess) e0 = hipStreamSynchronize(st);
-        if (e0 != hipSuccess) return e0;
-        if ((long long)L > g.rrec_cap) {
-            if (g.dev.rrec) (void)hipFree(g.dev.rrec);
-            g.dev.rrec = nullptr;
-            long long cap = std::min<long long>((long long)g.cap_l, (long long)L + L / 4 + 1024);
-            e0 = hipMalloc(&g.dev.rrec, sizeof(float4) * 2 * (size_t)cap);
-            if (e0 != hipSuccess) {
-                g.rrec_cap = 0;
-                return e0;
-            }
-            g.rrec_cap = cap;
-        }
-    }
-    kbeg(g, st, "k_recon_prep");
-    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_recon_prep<DD>, dim3(g.chunk_blocks), dim3(256), 0, st,
-                                            g.dev, G));
-    kend(g, st);
-    kbeg(g, st, "k_scan_keys");
-    hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, st, g.dev, g.chunk_blocks);
-    kend(g, st);
-    int nk = 0;
-    hipError_t e = hipMemcpyAsync(&nk, &g.dev.cnt->n_keys, sizeof(int), hipMemcpyDeviceToHost, st);
-    if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return e;
-    g.last_n_keys = nk;
-    if (nk > g.keys_cap) {
-        if (g.keys) (void)hipFree(g.keys);
-        if (g.keys_alt) (void)hipFree(g.keys_alt);
-        g.keys = g.keys_alt = nullptr;
-        long long cap = (long long)nk + nk / 4 + 1024;
-        e = hipMalloc(&g.keys, sizeof(unsigned long long) * cap);
+    const long long L = g.host.n_leaves;
+    if (L > g.rrec_cap) {
+        if (g.dev.rrec) (void)hipFree(g.dev.rrec);
+        g.dev.rrec = nullptr;
+        g.rrec_cap = 0;
+        const long long cap = std::min<long long>((long long)g.cap_l, L + L / 4 + 1024);
+        hipError_t e = hipMalloc(&g.dev.rrec, sizeof(float4) * 2 * (size_t)cap);
         if (e != hipSuccess) return e;
-        e = hipMalloc(&g.keys_alt, sizeof(unsigned long long) * cap);
-        if (e != hipSuccess) return e;
-        g.keys_cap = cap;
+        g.rrec_cap = cap;
     }
-    long long tiles = 1ll << (2 * G.tbits);
+    const long long kb = tiles_per_leaf(g, G) * std::max<long long>(L, 1);
+    g.recon_bounded = kb <= 0x7fffffffLL && kb * 16 <= kKeyBoundBytes;
+    if (g.recon_bounded && kb > g.keys_cap) {
+        hipError_t e = grow_keys(g, kb);
+        if (e != hipSuccess) return e;
+    }
+    const long long tiles = 1ll << (2 * G.tbits);
     if (tiles > g.tiles_cap) {
         if (g.tile_start) (void)hipFree(g.tile_start);
         if (g.tile_end) (void)hipFree(g.tile_end);
-        e = hipMalloc(&g.tile_start, sizeof(int) * tiles);
-        if (e != hipSuccess) return e;
-        e = hipMalloc(&g.tile_end, sizeof(int) * tiles);
+        g.tile_start = g.tile_end = nullptr;
+        g.tiles_cap = 0;
+        hipError_t e = hipMalloc(&g.tile_start, sizeof(int) * tiles);
+        if (e == hipSuccess) e = hipMalloc(&g.tile_end, sizeof(int) * tiles);
         if (e != hipSuccess) return e;
         g.tiles_cap = tiles;
     }
     if (!g.rs_hist) {
         // histograms plus the block sums of their scan
-        e = hipMalloc(&g.rs_hist, sizeof(unsigned) * ((1 << kSortMaxBits) * kSortBlocks + 1024));
+        hipError_t e = hipMalloc(&g.rs_hist, sizeof(unsigned) * ((1 << kSortMaxBits) * kSortBlocks + 1024));
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+// Per-leaf records, tile counts and their scan.  Stream-ordered when the key
+// buffers hold the bound; otherwise the key count is read back (a
+// synchronisation) and the buffers grown to it.
+hipError_t recon_prepare(Grid& g, int W, int H, hipStream_t st) {
+    hipError_t e = recon_reserve(g, W, H);
+    if (e != hipSuccess) return e;
+    ReconGeo G = make_geo(g, W, H);
+    g.recon_pixels = (long long)W * H;
+    kbeg(g, st, "k_recon_prep");
+    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_recon_prep<DD>, dim3(g.chunk_blocks), dim3(256), 0, st,
+                                            g.dev, G));
+    kend(g, st);
+    if (!g.recon_bounded) {
+        kbeg(g, st, "k_scan_keys");
+        hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, st, g.dev, g.chunk_blocks,
+                           (long long)0x7fffffff);
+        kend(g, st);
+        int nk = 0;
+        e = hipMemcpyAsync(&nk, &g.dev.cnt->n_keys, sizeof(int), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return e;
+        if (nk > g.keys_cap) {
+            e = grow_keys(g, (long long)nk + nk / 4 + 1024);
+            if (e != hipSuccess) return e;
+        }
+    } else {
+        kbeg(g, st, "k_scan_keys");
+        hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, st, g.dev, g.chunk_blocks, g.keys_cap);
+        kend(g, st);
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_integrate_gather(Grid& g, float* pixels, int W, int H, hipStream_t st,
@@ -738,21 +788,18 @@ hipError_t launch_integrate_gather(Grid& g, float* pixels, int W, int H, hipStre
                        g.keys_cap);
     kend(g, st);
     if (ev_sort0) (void)hipEventRecord(ev_sort0, st);
+    // the sorted keys end in g.keys or g.keys_alt by the parity of the pass
+    // count; the buffers keep their roles, so a captured graph stays valid
     unsigned long long* sorted = radix_sort_keys(g, 32, 32 + 2 * G.tbits, st);
+    g.keys_sorted = sorted;
     if (ev_sort1) (void)hipEventRecord(ev_sort1, st);
-    if (sorted != g.keys) {  // keep the sorted keys in g.keys for inspection
-        unsigned long long* t = g.keys;
-        g.keys = g.keys_alt;
-        g.keys_alt = t;
-    }
     const long long tiles = 1ll << (2 * G.tbits);
     kbeg(g, st, "k_tile_ranges");
     launch_fill_i32(g.tile_start, 0, tiles, st);
     launch_fill_i32(g.tile_end, 0, tiles, st);
-    const long long nk = g.last_n_keys;
-    int blocks = (int)std::min<long long>((nk + 255) / 256, 8192);
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_tile_ranges, dim3(blocks), dim3(256), 0, st, g.keys, &g.dev.cnt->n_keys,
+    int blocks = (int)std::min<long long>((g.keys_cap + 255) / 256, 8192);  // grid-stride over
+    if (blocks < 1) blocks = 1;                                              // the device count
+    hipLaunchKernelGGL(k_tile_ranges, dim3(blocks), dim3(256), 0, st, sorted, &g.dev.cnt->n_keys,
                        g.tile_start, g.tile_end);
     kend(g, st);
     const int ntiles = G.tilesX * G.tilesY;
@@ -761,11 +808,11 @@ hipError_t launch_integrate_gather(Grid& g, float* pixels, int W, int H, hipStre
     kbeg(g, st, "k_gather");
     if (gs == 64)
         hipLaunchKernelGGL(k_gather<64>, dim3((ntiles + tiles_per_block - 1) / tiles_per_block),
-                           dim3(256), 0, st, g.dev, G, g.keys, g.tile_start, g.tile_end,
+                           dim3(256), 0, st, g.dev, G, sorted, g.tile_start, g.tile_end,
                            (float4*)pixels);
     else
         hipLaunchKernelGGL(k_gather<16>, dim3((ntiles + tiles_per_block - 1) / tiles_per_block),
-                           dim3(256), 0, st, g.dev, G, g.keys, g.tile_start, g.tile_end,
+                           dim3(256), 0, st, g.dev, G, sorted, g.tile_start, g.tile_end,
                            (float4*)pixels);
     kend(g, st);
     return hipGetLastError();

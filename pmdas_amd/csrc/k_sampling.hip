@@ -93,6 +93,7 @@ __device__ __forceinline__ void initial_counters_body(DevCounters* c, int L0) {
     c->stale = 0;
     c->stale_vals = 0;
     c->carry_bits = 0;
+    c->guard_cap = 0;
 }
 __global__ void k_initial_counters(DevCounters* c, int L0) { initial_counters_body(c, L0); }
 
@@ -102,11 +103,15 @@ __global__ void k_reset_errors(DevCounters* c, int standalone) {
     if (standalone) {
         c->done = 0;
         c->limit = 0x7fffffff;
+        c->guard_cap = 0;
     }
     c->gmax_bits = 0;
 }
 
-__global__ void k_set_limit(DevCounters* c, int limit) { c->limit = limit; }
+__global__ void k_set_limit(DevCounters* c, int limit, int guard_cap) {
+    c->limit = limit;
+    c->guard_cap = guard_cap;
+}
 
 // Batched counter kernels: one thread per image.
 __global__ void k_initial_counters_b(const DevGrid* grids, int n, const int* limits) {
@@ -116,6 +121,7 @@ __global__ void k_initial_counters_b(const DevGrid* grids, int n, const int* lim
     DevCounters* c = a.cnt;
     initial_counters_body(c, 1 << (a.b * a.D + 2 * a.e));
     c->limit = limits[t];
+    c->guard_cap = limits[t] >= a.cap_s ? a.cap_s : 0;
 }
 
 // Re-arms the pass kernels for a standalone AdaptiveSampling call.
@@ -235,7 +241,7 @@ __device__ __forceinline__ void err_reduce_body(const DevGrid& a, int G) {
     __shared__ unsigned sm[16];
     __shared__ int ss[16], sv[16];
     if (a.cnt->done) return;
-    if (a.cnt->n_samples >= a.cnt->limit) {  // the loop stops before this pass
+    if (loop_limit_reached(a.cnt)) {  // the loop stops before this pass
         if (threadIdx.x == 0) a.cnt->done = 1;
         return;
     }
@@ -428,7 +434,7 @@ __device__ __forceinline__ void scan_counts_body(const DevGrid& a, int G) {
             cn->carry_bits = tm;
             cn->gmax_bits = tm;  // the next pass's max starts at the carried one
             const int p = cn->passes;
-            if (p < kMaxPassStats) a.pstats[p] = PassStat{L, N, ts, tl, tc, cn->stale, cn->stale_vals, 0};
+            a.pstats[p & (kMaxPassStats - 1)] = PassStat{L, N, ts, tl, tc, cn->stale, cn->stale_vals, 0};
             cn->passes = p + 1;
         }
         cn->sel_cnt = 0;
@@ -730,7 +736,7 @@ __global__ __launch_bounds__(256) void k_compact_place_b(const DevGrid* grids) {
 // (error -1, kdgrid.cu:466-469) the placement left out.
 __device__ __forceinline__ void mark_last_children_body(const DevGrid& a, int blk, int nblk) {
     const DevCounters* c = a.cnt;
-    if (c->status != 0 || c->n_samples < c->limit) return;
+    if (c->status != 0 || !loop_limit_reached(c)) return;
     const int nn = c->new_nodes, L_old = c->n_leaves_old;
     for (int t = blk * 256 + threadIdx.x; t < nn; t += nblk * 256) {
         a.err[a.split_list[t]] = -1.0f;
@@ -800,7 +806,10 @@ hipError_t launch_reset_errors(Grid& g, int standalone, hipStream_t st) {
 }
 
 hipError_t launch_set_limit(Grid& g, int limit, hipStream_t st) {
-    hipLaunchKernelGGL(k_set_limit, dim3(1), dim3(1), 0, st, g.dev.cnt, limit);
+    // the capacity guard applies when the limit is the capacity (a target at
+    // or beyond max_samples); below it a pass that would overflow is an error
+    hipLaunchKernelGGL(k_set_limit, dim3(1), dim3(1), 0, st, g.dev.cnt, limit,
+                       limit >= g.cap_s ? g.cap_s : 0);
     return hipGetLastError();
 }
 

@@ -67,7 +67,7 @@ void free_grid(Grid& g) {
     for (hipEvent_t e : g.evpool) (void)hipEventDestroy(e);
     g.evpool.clear();
     d = mdas::DevGrid{};
-    g.keys = g.keys_alt = nullptr;
+    g.keys = g.keys_alt = g.keys_sorted = nullptr;
     g.tile_start = g.tile_end = nullptr;
     g.rs_hist = nullptr;
     g.d_scalar = nullptr;
@@ -83,11 +83,20 @@ int sync_counters(Grid& g, hipStream_t st) {
     return MDAS_OK;
 }
 
-void log_kv(Grid& g, const char* key, double v) {
+// One "KEY\nvalue" pair as the reference writes it (kdgrid.cu:670-671,
+// 713, 769-770, 816-817): sample counts as integers, times as the float that
+// hipEventElapsedTime gives, in the stream's default format.
+void log_line(Grid& g, const char* key, double v, bool count) {
+    if (g.log_fn) g.log_fn(g.log_user, key, v, count ? 1 : 0);
     if (g.log_path.empty()) return;
     std::ofstream f(g.log_path, std::ios::app);
-    f << key << "\n" << v << "\n";
+    if (count)
+        f << key << "\n" << (long long)v << std::endl;
+    else
+        f << key << "\n" << (float)v << std::endl;
 }
+void log_count(Grid& g, const char* key, long long n) { log_line(g, key, (double)n, true); }
+void log_time(Grid& g, const char* key, double ms) { log_line(g, key, ms, false); }
 
 float elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0.0f;
@@ -127,11 +136,32 @@ void collect_kernel_times(Grid& g, std::vector<std::pair<std::string, double>>* 
     g.t_integrand = get("k_eval");
 }
 
+// The initial cell's x / y extents, the largest any leaf has until an upload
+// (kdgrid.cu:158-164: 2^-(b+e) of the sample space along x and y).
+void set_initial_extents(Grid& g) {
+    g.ext_x = std::ldexp((double)g.sX, -(g.b + g.e));
+    g.ext_y = std::ldexp((double)g.sY, -(g.b + g.e));
+}
+
 }  // namespace
 
 namespace mdas {
 int set_error(int code, const std::string& msg) { return fail(code, msg); }
+void initial_extents(Grid& g) { set_initial_extents(g); }
 int sync_grid(Grid& g, hipStream_t st) { return sync_counters(g, st); }
+int drain_pass_stats(Grid& g, bool all) {
+    const int have = (int)g.pstats_host.size(), done = g.host.passes;
+    if (done <= have || (!all && done - have < kMaxPassStats / 2)) return MDAS_OK;
+    g.pstats_host.resize((size_t)done);
+    for (int p = have; p < done;) {  // contiguous stretches of the ring
+        const int r = p & (kMaxPassStats - 1);
+        const int n = std::min(done - p, kMaxPassStats - r);
+        HIP_TRY(hipMemcpy(&g.pstats_host[(size_t)p], g.dev.pstats + r, sizeof(PassStat) * (size_t)n,
+                          hipMemcpyDeviceToHost));
+        p += n;
+    }
+    return MDAS_OK;
+}
 }  // namespace mdas
 
 extern "C" {
@@ -235,7 +265,7 @@ int mdas_initial_sampling(mdas_handle h, void* stream) {
     hipStream_t st = S(stream);
     const int L0 = 1 << (g.b * g.D + 2 * g.e);
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    const bool logging = !g.log_path.empty();
+    const bool logging = g.logging();
     if (logging) {
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
@@ -246,6 +276,7 @@ int mdas_initial_sampling(mdas_handle h, void* stream) {
     // the new leaf's slot, which gives the same result without the fill.
     HIP_TRY(mdas::launch_initial(g, L0, st));
     g.initialized = true;
+    set_initial_extents(g);
     g.host.n_leaves = L0;
     g.host.n_samples = 4 * L0;
     g.host.new_samples = 4 * L0;
@@ -254,8 +285,8 @@ int mdas_initial_sampling(mdas_handle h, void* stream) {
     if (logging) {
         HIP_TRY(hipEventRecord(e1, st));
         HIP_TRY(hipEventSynchronize(e1));
-        log_kv(g, "INITIAL SAMPLES", 4.0 * L0);
-        log_kv(g, "INITIAL SAMPLING TIME", elapsed(e0, e1));
+        log_count(g, "INITIAL SAMPLES", 4LL * L0);
+        log_time(g, "INITIAL SAMPLING TIME", elapsed(e0, e1));
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     }
@@ -268,7 +299,7 @@ int mdas_compute_errors(mdas_handle h, void* stream) {
     if (!g.initialized) return fail(MDAS_ERR_STATE, "InitialSampling has not been called");
     hipStream_t st = S(stream);
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    const bool logging = !g.log_path.empty();
+    const bool logging = g.logging();
     if (logging) {
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
@@ -279,7 +310,7 @@ int mdas_compute_errors(mdas_handle h, void* stream) {
     if (logging) {
         HIP_TRY(hipEventRecord(e1, st));
         HIP_TRY(hipEventSynchronize(e1));
-        log_kv(g, "COMPUTE ERRORS TIME", elapsed(e0, e1));
+        log_time(g, "COMPUTE ERRORS TIME", elapsed(e0, e1));
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     }
@@ -292,7 +323,7 @@ int mdas_adaptive_sampling(mdas_handle h, void* stream) {
     if (!g.initialized) return fail(MDAS_ERR_STATE, "InitialSampling has not been called");
     hipStream_t st = S(stream);
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    const bool logging = !g.log_path.empty();
+    const bool logging = g.logging();
     if (logging) {
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
@@ -318,8 +349,8 @@ int mdas_adaptive_sampling(mdas_handle h, void* stream) {
     if (logging) {
         HIP_TRY(hipEventRecord(e1, st));
         HIP_TRY(hipEventSynchronize(e1));
-        log_kv(g, "SAMPLES", g.host.new_samples);
-        log_kv(g, "ADAPTIVE SAMPLING TIME", elapsed(e0, e1));
+        log_count(g, "SAMPLES", g.host.new_samples);
+        log_time(g, "ADAPTIVE SAMPLING TIME", elapsed(e0, e1));
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     }
@@ -346,7 +377,7 @@ int mdas_integrate(mdas_handle h, float* pixels, int w, int hgt, void* stream) {
     int rc = check_image(g, w, hgt);
     if (rc) return rc;
     hipStream_t st = S(stream);
-    const bool logging = !g.log_path.empty();
+    const bool logging = g.logging();
     const bool ev = g.timing || logging;
     hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
     if (ev) {
@@ -361,14 +392,16 @@ int mdas_integrate(mdas_handle h, float* pixels, int w, int hgt, void* stream) {
         g.t_recon = elapsed(e[0], e[3]);
         g.t_sort = elapsed(e[1], e[2]);
         for (auto& x : e) (void)hipEventDestroy(x);
+        int rc2 = sync_counters(g, st);  // the key count, for the byte model
+        if (rc2) return rc2;
+        if (g.host.recon_overflow)
+            return fail(MDAS_ERR_CAPACITY, "reconstruction keys exceed their buffer");
         collect_kernel_times(g);
     }
     if (logging) {
-        int rc2 = sync_counters(g, st);
-        if (rc2) return rc2;
-        log_kv(g, "TOTAL SAMPLES", g.host.n_samples);
-        log_kv(g, "CONSTRUCT TIME", g.t_sort);
-        log_kv(g, "INTEGRATE TIME", g.t_recon);
+        log_count(g, "TOTAL SAMPLES", g.host.n_samples);
+        log_time(g, "CONSTRUCT TIME", g.t_sort);
+        log_time(g, "INTEGRATE TIME", g.t_recon);
     }
     return MDAS_OK;
 }
@@ -447,6 +480,22 @@ int mdas_upload_state(mdas_handle h, int n, int l, const float* coords, const fl
     Grid& g = h->g;
     if (n < 0 || l < 0 || n > g.cap_s || l > g.cap_l) return fail(MDAS_ERR_INVALID, "state exceeds capacity");
     const int D = g.D;
+    if ((idx != nullptr) != (box != nullptr))
+        return fail(MDAS_ERR_INVALID, "slots and boxes are uploaded together");
+    // what the kernels rely on, checked before anything is copied: a slot
+    // index below n, the occupied slots first (a pass fills slots in order
+    // and a split writes both children from slot 0)
+    for (int i = 0; idx && i < l; ++i) {
+        int c = 0;
+        for (int q = 0; q < 4; ++q) {
+            const int k = idx[4 * (size_t)i + q];
+            if (k >= n)
+                return fail(MDAS_ERR_INVALID, "leaf " + std::to_string(i) + ": sample index beyond the samples");
+            if (k >= 0 && c != q)
+                return fail(MDAS_ERR_INVALID, "leaf " + std::to_string(i) + ": an occupied slot after an empty one");
+            c += k >= 0 ? 1 : 0;
+        }
+    }
     HIP_TRY(hipDeviceSynchronize());
     if (coords) HIP_TRY(hipMemcpy(g.dev.coords, coords, sizeof(float) * (size_t)n * D, hipMemcpyHostToDevice));
     if (values) HIP_TRY(hipMemcpy(g.dev.values, values, sizeof(float) * (size_t)n * 3, hipMemcpyHostToDevice));
@@ -456,21 +505,26 @@ int mdas_upload_state(mdas_handle h, int n, int l, const float* coords, const fl
         memcpy(s.data(), seed, sizeof(uint32_t) * (size_t)l);
         HIP_TRY(hipMemcpy(g.dev.seed, s.data(), sizeof(uint32_t) * s.size(), hipMemcpyHostToDevice));
     }
-    if (idx || box) {
-        if (!idx || !box) return fail(MDAS_ERR_INVALID, "slots and boxes are uploaded together");
+    if (idx) {
         const int Q = mdas::leaf_quads(D);
         std::vector<float> rec((size_t)l * 4 * Q, 0.0f);
         std::vector<unsigned char> f((size_t)l);
         for (int i = 0; i < l; ++i) {
+            int c = 0;
+            for (int q = 0; q < 4; ++q) c += idx[4 * (size_t)i + q] >= 0 ? 1 : 0;
             float* r = &rec[(size_t)i * 4 * Q];
             memcpy(r, idx + 4 * (size_t)i, 16);
             memcpy(r + 4, box + (size_t)i * 2 * D, sizeof(float) * 2 * D);
-            int c = 0;
-            for (int q = 0; q < 4; ++q) c += idx[4 * i + q] >= 0 ? 1 : 0;
             f[i] = (unsigned char)c;
         }
         HIP_TRY(hipMemcpy(g.dev.leaf, rec.data(), sizeof(float) * rec.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(g.dev.flags, f.data(), f.size(), hipMemcpyHostToDevice));
+        g.ext_x = g.ext_y = 0.0;
+        for (int i = 0; i < l; ++i) {
+            const float* bx = box + (size_t)i * 2 * D;
+            g.ext_x = std::max(g.ext_x, (double)(bx[D] - bx[0]));
+            g.ext_y = std::max(g.ext_y, (double)(bx[D + 1] - bx[1]));
+        }
     }
     if (idx || coords) {
         HIP_TRY(mdas::launch_build_pts(g, l, 0));
@@ -500,11 +554,15 @@ int mdas_set_values(mdas_handle h, int offset, int count, const float* hv) {
 int mdas_get_recon_keys(mdas_handle h, uint64_t* out, long long capacity, long long* n) {
     if (!h) return fail(MDAS_ERR_INVALID, "null handle");
     Grid& g = h->g;
+    int rc = sync_counters(g, nullptr);
+    if (rc) return rc;
     HIP_TRY(hipDeviceSynchronize());
-    if (n) *n = g.last_n_keys;
-    if (out && g.keys) {
-        long long m = std::min(capacity, g.last_n_keys);
-        HIP_TRY(hipMemcpy(out, g.keys, sizeof(uint64_t) * (size_t)m, hipMemcpyDeviceToHost));
+    if (g.host.recon_overflow) return fail(MDAS_ERR_CAPACITY, "reconstruction keys exceeded their buffer");
+    const long long nk = g.host.n_keys;
+    if (n) *n = nk;
+    if (out && g.keys_sorted) {
+        long long m = std::min(capacity, nk);
+        HIP_TRY(hipMemcpy(out, g.keys_sorted, sizeof(uint64_t) * (size_t)m, hipMemcpyDeviceToHost));
     }
     return MDAS_OK;
 }
@@ -521,38 +579,32 @@ int mdas_validate(mdas_handle h, int* valid) {
     std::vector<int> idx((size_t)L * 4);
     rc = mdas_download_state(h, coords.data(), nullptr, idx.data(), box.data(), nullptr, nullptr);
     if (rc) return rc;
+    // the reference's four conditions (kdgrid.cu:876-956) and a slot index
+    // below N (the reference would read past its arrays)
     bool ok = true;
     std::vector<int> hist((size_t)N, 0);
     double vol = 0.0;
     long long total = 0;
-    for (long long i = 0; i < L && ok; ++i) {
+    for (long long i = 0; i < L; ++i) {
         const float* mn = &box[(size_t)i * 2 * D];
         const float* mx = mn + D;
-        int n = 0;
-        bool prefix = true;
         for (int q = 0; q < 4; ++q) {
-            int k = idx[(size_t)i * 4 + q];
-            if (k >= 0) {
-                if (!prefix || k >= N) { ok = false; break; }
-                for (int d = 0; d < D; ++d) {  // aabb.h Contains
-                    float c = coords[(size_t)k * D + d];
-                    if (c < mn[d] || c > mx[d]) ok = false;
-                }
-                hist[k]++;
-                ++n;
-            } else {
-                if (k != ~D) ok = false;
-                prefix = false;
+            const int k = idx[(size_t)i * 4 + q];
+            if (k < 0) continue;  // an empty slot
+            if (k >= N) {
+                ok = false;
+                continue;
             }
+            for (int d = 0; d < D; ++d) {  // aabb.h:57-62 Contains
+                const float c = coords[(size_t)k * D + d];
+                if (c < mn[d] || c > mx[d]) ok = false;
+            }
+            hist[k]++;
+            ++total;
         }
-        if (n < 2) ok = false;
         double v = 1.0;
-        for (int d = 0; d < D; ++d) {
-            if (!(mn[d] < mx[d])) ok = false;
-            v *= (double)(mx[d] - mn[d]);
-        }
+        for (int d = 0; d < D; ++d) v *= (double)(mx[d] - mn[d]);
         vol += v;
-        total += n;
     }
     const double root = (double)g.sX * (double)g.sY;
     if (std::fabs(vol - root) > 1.0e-2 * root) ok = false;  // kdgrid.cu:909-914
@@ -651,6 +703,13 @@ int mdas_render_qmc(mdas_scene_handle s, int w, int hgt, int spp, int first_samp
 }
 
 // ---------------------------------------------------------------- the loop --
+int mdas_set_log_sink(mdas_handle h, mdas_log_fn fn, void* user) {
+    if (!h) return fail(MDAS_ERR_INVALID, "null handle");
+    h->g.log_fn = fn;
+    h->g.log_user = user;
+    return MDAS_OK;
+}
+
 int mdas_set_timing(mdas_handle h, int enable) {
     if (!h) return fail(MDAS_ERR_INVALID, "null handle");
     h->g.timing = enable != 0;
@@ -667,7 +726,7 @@ int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream)
     const long long limit = std::min<long long>(target, g.max_samples);
     const int B = std::max(1, g.passes_per_sync);
     // the log needs per-pass times: logging times the run like mdas_set_timing
-    const bool logging = !g.log_path.empty();
+    const bool logging = g.logging();
     const bool timing_was = g.timing;
     if (logging) g.timing = true;
     struct Restore {
@@ -679,11 +738,13 @@ int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream)
     g.krecs.clear();
     g.evnext = 0;
     g.t_recon = g.t_sort = 0.0;
+    g.pstats_host.clear();
     mdas::kbeg(g, st, "k_initial");
     HIP_TRY(mdas::launch_initial(g, L0, st));
     HIP_TRY(mdas::launch_set_limit(g, (int)limit, st));
     mdas::kend(g, st);
     g.initialized = true;
+    set_initial_extents(g);
     mdas::kbeg(g, st, "k_eval");
     HIP_TRY(mdas::launch_eval(g, s->dev, 0, 4 * L0, false, st));
     mdas::kend(g, st);
@@ -713,13 +774,16 @@ int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream)
             mdas::kend(g, st);
         }
         int rc = sync_counters(g, st);
+        if (rc == MDAS_OK) rc = mdas::drain_pass_stats(g, false);
         if (rc) return rc;
         if (g.host.done || g.host.status) break;
         // every pass selects the leaf of maximum error (its ratio is 1), so
         // each block adds samples and the loop ends at the limit; a block that
-        // adds none means the errors are not finite (e.g. a NaN integrand)
+        // adds none means no leaf passed the selection test: the maximum error
+        // is not finite (e.g. a NaN integrand) or zero (scores that underflow)
         if (g.host.n_samples == last_n)
-            return fail(MDAS_ERR_STATE, "a block of passes added no sample (non-finite errors)");
+            return fail(MDAS_ERR_STATE, "a block of passes added no sample (no leaf selected: "
+                                        "non-finite or zero maximum error)");
         last_n = g.host.n_samples;
     }
     mdas::kbeg(g, st, "k_mark_last");
@@ -728,24 +792,24 @@ int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream)
     g.passes = g.host.passes;
     std::vector<std::pair<std::string, double>> seq;
     collect_kernel_times(g, logging ? &seq : nullptr);
-    g.pstats_host.assign((size_t)std::min(g.passes, mdas::kMaxPassStats), mdas::PassStat{});
-    if (!g.pstats_host.empty())
-        HIP_TRY(hipMemcpy(g.pstats_host.data(), g.dev.pstats, sizeof(mdas::PassStat) * g.pstats_host.size(),
-                          hipMemcpyDeviceToHost));
+    {
+        int rc = mdas::drain_pass_stats(g, true);
+        if (rc) return rc;
+    }
     if (logging && seq.size() >= 2) {
         // the app's and the library's keys in the order they write them
         // (kdgrid.cu:670-671, 713, 769-770; optixMotionBlur.cpp:774, 788):
         // the initial grid, its integrand launch, then per pass the error
         // estimate, the new samples, the adaptive step and the integrand
-        log_kv(g, "INITIAL SAMPLES", 4.0 * L0);
-        log_kv(g, "INITIAL SAMPLING TIME", seq[0].second);
-        log_kv(g, "TRACE TIME", seq[1].second);
+        log_count(g, "INITIAL SAMPLES", 4LL * L0);
+        log_time(g, "INITIAL SAMPLING TIME", seq[0].second);
+        log_time(g, "TRACE TIME", seq[1].second);
         size_t k = 2;
         for (int p = 0; p < g.passes && k + 4 < seq.size(); ++p, k += 5) {
-            log_kv(g, "COMPUTE ERRORS TIME", seq[k].second);
-            log_kv(g, "SAMPLES", p < (int)g.pstats_host.size() ? g.pstats_host[p].sel : 0);
-            log_kv(g, "ADAPTIVE SAMPLING TIME", seq[k + 1].second + seq[k + 2].second + seq[k + 3].second);
-            log_kv(g, "TRACE TIME", seq[k + 4].second);
+            log_time(g, "COMPUTE ERRORS TIME", seq[k].second);
+            log_count(g, "SAMPLES", p < (int)g.pstats_host.size() ? g.pstats_host[p].sel : 0);
+            log_time(g, "ADAPTIVE SAMPLING TIME", seq[k + 1].second + seq[k + 2].second + seq[k + 3].second);
+            log_time(g, "TRACE TIME", seq[k + 4].second);
         }
     }
     if (g.host.status) {
@@ -804,7 +868,7 @@ static double kernel_bytes(const Grid& g, const std::string& k) {
                                                             // records and their points, errors, split entry
     }
     const double L = (double)g.host.n_leaves, N = (double)g.host.n_samples;
-    const double K = (double)g.last_n_keys;
+    const double K = (double)g.host.n_keys;
     if (k == "k_recon_prep") return L * (LB + 32.0) + 12.0 * N;
     if (k == "k_scan_keys") return (16.0 + 8.0) * g.chunk_blocks;
     if (k == "k_recon_emit") return L * 16.0 + K * 8.0;

@@ -6,6 +6,7 @@
 // stopping rule has fired.  Each image follows exactly the single-image
 // algorithm, so its result equals mdas_run on that image alone.
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 #include <string.h>
 
 #include <algorithm>
@@ -23,7 +24,7 @@ struct Group {
     mdas::DevGrid* d_grids = nullptr;    // device copies of the descriptors
     const void** d_params = nullptr;     // device array of scene parameter blocks
     int* d_limits = nullptr;
-    int* d_status = nullptr;             // (done, status, samples) per member
+    int* d_status = nullptr;             // (done, status, samples, passes) per member
     mdas::LearnDev* d_learn = nullptr;   // per member, when any member is in learned mode
     int blocks = 0;
     hipGraph_t graph = nullptr;
@@ -40,9 +41,10 @@ struct Group {
 __global__ void k_batch_status(const mdas::DevGrid* grids, int n, int* out) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) {
-        out[3 * t] = grids[t].cnt->done;
-        out[3 * t + 1] = grids[t].cnt->status;
-        out[3 * t + 2] = grids[t].cnt->n_samples;
+        out[4 * t] = grids[t].cnt->done;
+        out[4 * t + 1] = grids[t].cnt->status;
+        out[4 * t + 2] = grids[t].cnt->n_samples;
+        out[4 * t + 3] = grids[t].cnt->passes;
     }
 }
 
@@ -57,7 +59,22 @@ struct mdas_batch {
     double last_ms = 0.0;
     int last_blocks = 0;
     int shape = 8192;  // workgroups per launch in all (mdas_batch_autotune)
+    // the images' reconstructions as one captured graph (mdas_batch_integrate),
+    // with every pointer and size its launches carry
+    std::vector<unsigned long long> recon_key;
+    hipGraph_t recon_graph = nullptr;
+    hipGraphExec_t recon_exec = nullptr;
 };
+
+namespace {
+void drop_recon_graph(mdas_batch* b) {
+    if (b->recon_exec) (void)hipGraphExecDestroy(b->recon_exec);
+    if (b->recon_graph) (void)hipGraphDestroy(b->recon_graph);
+    b->recon_exec = nullptr;
+    b->recon_graph = nullptr;
+    b->recon_key.clear();
+}
+}  // namespace
 
 extern "C" {
 
@@ -97,7 +114,7 @@ int mdas_batch_create(mdas_handle* grids, mdas_scene_handle* scenes, int n, mdas
         hipError_t e = hipMalloc(&gr.d_grids, sizeof(mdas::DevGrid) * m);
         if (e == hipSuccess) e = hipMalloc((void**)&gr.d_params, sizeof(void*) * m);
         if (e == hipSuccess) e = hipMalloc(&gr.d_limits, sizeof(int) * m);
-        if (e == hipSuccess) e = hipMalloc(&gr.d_status, sizeof(int) * 3 * m);
+        if (e == hipSuccess) e = hipMalloc(&gr.d_status, sizeof(int) * 4 * m);
         if (e == hipSuccess) e = hipMemcpy(gr.d_grids, dg.data(), sizeof(mdas::DevGrid) * m, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(gr.d_params, dp.data(), sizeof(void*) * m, hipMemcpyHostToDevice);
         if (e == hipSuccess && learned) e = hipMalloc(&gr.d_learn, sizeof(mdas::LearnDev) * m);
@@ -125,6 +142,7 @@ int mdas_batch_destroy(mdas_batch_handle b) {
         if (g.d_status) (void)hipFree(g.d_status);
         if (g.d_learn) (void)hipFree(g.d_learn);
     }
+    drop_recon_graph(b);
     if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
     delete b;
     return MDAS_OK;
@@ -159,7 +177,11 @@ int mdas_batch_run(mdas_batch_handle b, const long long* targets, void* stream, 
         B_TRY(hipMemcpyAsync(g.d_limits, limits[gi].data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
         B_TRY(mdas::launch_initial_b(g.D, g.d_grids, m, g.blocks, g.d_limits, st));
         B_TRY(mdas::launch_eval_b(g.d_grids, g.d_params, m, g.blocks, st));
-        for (int k = 0; k < m; ++k) b->grids[g.members[k]]->g.initialized = true;
+        for (int k = 0; k < m; ++k) {
+            b->grids[g.members[k]]->g.initialized = true;
+            mdas::initial_extents(b->grids[g.members[k]]->g);
+            b->grids[g.members[k]]->g.pstats_host.clear();
+        }
     }
     std::vector<char> finished(b->groups.size(), 0);
     std::vector<int> last_n(n, -1);  // samples of each image after the last block
@@ -197,24 +219,29 @@ int mdas_batch_run(mdas_batch_handle b, const long long* targets, void* stream, 
             if (finished[gi]) continue;
             Group& g = b->groups[gi];
             const int m = (int)g.members.size();
-            std::vector<int> s(3 * m);
-            B_TRY(hipMemcpy(s.data(), g.d_status, sizeof(int) * 3 * m, hipMemcpyDeviceToHost));
+            std::vector<int> s(4 * m);
+            B_TRY(hipMemcpy(s.data(), g.d_status, sizeof(int) * 4 * m, hipMemcpyDeviceToHost));
             bool done = true;
             for (int k = 0; k < m; ++k) {
-                if (s[3 * k + 1] != 0) {
+                if (s[4 * k + 1] != 0) {
                     finished[gi] = 1;
-                    return mdas::set_error(s[3 * k + 1], "image " + std::to_string(g.members[k]) +
+                    return mdas::set_error(s[4 * k + 1], "image " + std::to_string(g.members[k]) +
                                                              ": adaptive pass would exceed max_samples");
                 }
                 // every pass selects the leaf of maximum error, so a block of
                 // passes that adds no sample means the errors are not finite
                 int& prev = last_n[g.members[k]];
-                if (s[3 * k] == 0 && s[3 * k + 2] == prev)
+                if (s[4 * k] == 0 && s[4 * k + 2] == prev)
                     return mdas::set_error(MDAS_ERR_STATE, "image " + std::to_string(g.members[k]) +
                                                                ": a block of passes added no sample "
-                                                               "(non-finite errors)");
-                prev = s[3 * k + 2];
-                done = done && s[3 * k] != 0;
+                                                               "(no leaf selected: non-finite or zero "
+                                                               "maximum error)");
+                prev = s[4 * k + 2];
+                done = done && s[4 * k] != 0;
+                mdas::Grid& gk = b->grids[g.members[k]]->g;  // rows before the ring wraps
+                gk.host.passes = s[4 * k + 3];
+                const int rc = mdas::drain_pass_stats(gk, false);
+                if (rc) return rc;
             }
             finished[gi] = done ? 1 : 0;
             all = all && done;
@@ -228,13 +255,86 @@ int mdas_batch_run(mdas_batch_handle b, const long long* targets, void* stream, 
         int rc = mdas::sync_grid(gg, st);
         if (rc) return rc;
         gg.passes = gg.host.passes;
-        gg.pstats_host.assign((size_t)std::min(gg.passes, mdas::kMaxPassStats), mdas::PassStat{});
-        if (!gg.pstats_host.empty())
-            B_TRY(hipMemcpy(gg.pstats_host.data(), gg.dev.pstats,
-                            sizeof(mdas::PassStat) * gg.pstats_host.size(), hipMemcpyDeviceToHost));
+        rc = mdas::drain_pass_stats(gg, true);
+        if (rc) return rc;
     }
     b->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     b->last_blocks = blocks_run;
+    return MDAS_OK;
+}
+
+// Reconstructs every image into its own float4 image (kdgrid.cu:781-820 per
+// image).  With use_graph the reconstructions of all images are captured once
+// into a hipGraph and replayed while every image's size, gather tile, output
+// and reconstruction buffers stay the same; the buffers are sized before the
+// capture from each image's leaf count, so nothing in the graph allocates or
+// waits for the host.  Stream-ordered: the caller synchronises.
+int mdas_batch_integrate(mdas_batch_handle b, float* const* pixels, const int* widths,
+                         const int* heights, void* stream, int use_graph) {
+    if (!b || !pixels || !widths || !heights) return mdas::set_error(MDAS_ERR_INVALID, "null argument");
+    const int n = (int)b->grids.size();
+    for (int i = 0; i < n; ++i) {
+        const mdas::Grid& g = b->grids[i]->g;
+        if (!pixels[i]) return mdas::set_error(MDAS_ERR_INVALID, "null image " + std::to_string(i));
+        if (widths[i] <= 0 || heights[i] <= 0 || widths[i] > 65535 || heights[i] > 65535)
+            return mdas::set_error(MDAS_ERR_INVALID, "image size out of range at " + std::to_string(i));
+        if (!g.initialized) return mdas::set_error(MDAS_ERR_STATE, "image " + std::to_string(i) + " has no samples");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (use_graph && !st) {
+        if (!b->own_stream) B_TRY(hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking));
+        st = b->own_stream;
+    }
+    bool bounded = true;
+    for (int i = 0; i < n; ++i) {
+        mdas::Grid& g = b->grids[i]->g;
+        B_TRY(mdas::recon_reserve(g, widths[i], heights[i]));
+        bounded = bounded && g.recon_bounded;
+    }
+    if (!use_graph || !bounded) {  // one image after the other (may synchronise)
+        for (int i = 0; i < n; ++i) {
+            const int rc = mdas_integrate(b->grids[i], pixels[i], widths[i], heights[i], st);
+            if (rc) return rc;
+        }
+        return MDAS_OK;
+    }
+    std::vector<unsigned long long> key;
+    for (int i = 0; i < n; ++i) {
+        const mdas::Grid& g = b->grids[i]->g;
+        const unsigned long long k[] = {
+            (unsigned long long)(uintptr_t)pixels[i], (unsigned long long)widths[i],
+            (unsigned long long)heights[i], (unsigned long long)g.tile_w, (unsigned long long)g.tile_h,
+            (unsigned long long)g.chunk_blocks, (unsigned long long)(uintptr_t)g.dev.rrec,
+            (unsigned long long)(uintptr_t)g.keys, (unsigned long long)(uintptr_t)g.keys_alt,
+            (unsigned long long)g.keys_cap, (unsigned long long)(uintptr_t)g.tile_start,
+            (unsigned long long)(uintptr_t)g.tile_end, (unsigned long long)(uintptr_t)g.rs_hist};
+        key.insert(key.end(), k, k + sizeof(k) / sizeof(k[0]));
+    }
+    if (!b->recon_exec || key != b->recon_key) {
+        drop_recon_graph(b);
+        std::vector<char> timing(n);
+        for (int i = 0; i < n; ++i) {  // no timing events inside the graph
+            timing[i] = b->grids[i]->g.timing;
+            b->grids[i]->g.timing = false;
+        }
+        B_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t ce = hipSuccess;
+        for (int i = 0; i < n && ce == hipSuccess; ++i) {
+            mdas::Grid& g = b->grids[i]->g;
+            ce = mdas::recon_prepare(g, widths[i], heights[i], st);
+            if (ce == hipSuccess)
+                ce = mdas::launch_integrate_gather(g, pixels[i], widths[i], heights[i], st, nullptr, nullptr);
+        }
+        hipGraph_t graph = nullptr;
+        const hipError_t ee = hipStreamEndCapture(st, &graph);
+        for (int i = 0; i < n; ++i) b->grids[i]->g.timing = timing[i] != 0;
+        B_TRY(ce);
+        B_TRY(ee);
+        b->recon_graph = graph;
+        B_TRY(hipGraphInstantiate(&b->recon_exec, b->recon_graph, nullptr, nullptr, 0));
+        b->recon_key = key;
+    }
+    B_TRY(hipGraphLaunch(b->recon_exec, st));
     return MDAS_OK;
 }
 

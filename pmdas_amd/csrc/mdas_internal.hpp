@@ -27,6 +27,11 @@ struct DevCounters {
     int stale;           // leaves whose error was recomputed in this pass
     int stale_vals;      // sample values read for them
     unsigned carry_bits; // max error of the leaves the last adaptive pass did not split
+    int recon_overflow;  // keys counted by the last reconstruction when they exceeded the
+                         //   key buffer (then nothing was emitted), else 0
+    int guard_cap;       // device-side loop: no pass starts once N + L > guard_cap (the
+                         //   sample capacity when the target is at or beyond it; 0 = no
+                         //   guard: a smaller target, standalone calls)
 };
 
 // Per-pass statistics, one row per adaptive pass (for algorithmic byte counts):
@@ -45,7 +50,8 @@ struct KRec {
 
 static const int kChunkBlocks = 32768;  // max workgroups of the chunked leaf kernels
 static const int kErrBlocks = 8192;     // max workgroups of the error kernels
-static const int kMaxPassStats = 4096;
+static const int kMaxPassStats = 4096;  // device ring of per-pass rows (a power of two); the
+                                        // host drains it before it wraps
 
 // Float4 quads of one leaf record: the int4 of sample slots, then the box
 // (mn[0..D-1], mx[0..D-1]) padded to 16 bytes -- 48 bytes for D = 3, 4 and
@@ -130,17 +136,26 @@ struct Grid {
     DevCounters host{};  // mirror after the last synchronising call
     float gmax_host = 0.0f;
     std::string log_path;
+    // log sink (mdas_set_log_sink): each "KEY", value pair the reference
+    // writes to its std::ofstream; is_count marks integral values
+    void (*log_fn)(void* user, const char* key, double value, int is_count) = nullptr;
+    void* log_user = nullptr;
+    bool logging() const { return !log_path.empty() || log_fn != nullptr; }
     // reconstruction scratch
     unsigned long long* keys = nullptr;
     unsigned long long* keys_alt = nullptr;
+    unsigned long long* keys_sorted = nullptr;  // keys or keys_alt: the last sort's result
     long long keys_cap = 0;
     int* tile_start = nullptr;
     int* tile_end = nullptr;
     long long tiles_cap = 0;
     unsigned* rs_hist = nullptr;  // radix histogram [1024 * kSortBlocks] + scan sums
     int tile_w = 8, tile_h = 8;  // reconstruction tile (mdas_set_recon_tile / autotune)
-    long long last_n_keys = 0;
     long long rrec_cap = 0;       // leaves the reconstruction records can hold
+    bool recon_bounded = false;   // the key buffers hold the bound of the last reservation
+    // largest leaf extent along x and y (sample-space units): the initial cell's,
+    // which splits only shrink, or the largest of an uploaded state's boxes
+    double ext_x = 0.0, ext_y = 0.0;
     int* d_scalar = nullptr;      // small device scratch
     // timing: per-kernel events when enabled
     bool timing = false;
@@ -252,6 +267,7 @@ hipError_t launch_mlp_adam(float* p, float* m, float* v, const float* gpart, con
 hipError_t launch_feat_stats(const float* X, long long n, double* part, int nblocks,
                              hipStream_t st);
 static const int kMlpGradBlocks = 256;  // workgroups of the gradient kernel (4 waves each)
+hipError_t recon_reserve(Grid& g, int W, int H);
 hipError_t recon_prepare(Grid& g, int W, int H, hipStream_t st);
 
 }  // namespace mdas

@@ -88,11 +88,20 @@ __device__ __forceinline__ void store_leaf(const DevGrid& a, int i, int4 id, con
 // are always 0..n-1: slots fill in order and a split writes both children
 // from slot 0.
 // Stopping rule of the device-side loop (optixMotionBlur.cpp:781): no
-// further pass once N >= min(target, max_samples).  The error kernels of a
-// pass test it (every workgroup reads the same counters) and the reduction
-// after them raises `done` for the rest of the pass.
+// further pass once N >= min(target, max_samples).  When the limit is the
+// capacity itself (target >= max_samples) also none once a pass could
+// overflow it, N + L > max_samples (every leaf selected adds L samples; the
+// guard of the app's interactive loop, optixMotionBlur.cpp:723-724), so that
+// such a run ends cleanly instead of with a capacity error.
+// The error kernels of a pass test it (every workgroup reads the same
+// counters) and the reduction after them raises `done` for the rest of the
+// pass.
+__device__ __forceinline__ bool loop_limit_reached(const DevCounters* c) {
+    return c->n_samples >= c->limit ||
+           (c->guard_cap != 0 && (long long)c->n_samples + c->n_leaves > (long long)c->guard_cap);
+}
 __device__ __forceinline__ bool pass_stopped(const DevCounters* c) {
-    return c->done || c->n_samples >= c->limit;
+    return c->done || loop_limit_reached(c);
 }
 
 struct LeafSamples {

@@ -138,3 +138,40 @@ def device_name():
 
 def visible():
     return os.environ.get("HIP_VISIBLE_DEVICES", "")
+
+
+class Graph:
+    """Stream capture into a hipGraph (begin / end / replay) for callers that
+    enqueue library calls into a graph."""
+
+    def __init__(self, stream):
+        self.stream = stream
+        self.graph = None
+        self.exec = None
+
+    def begin(self):
+        # mode 1: hipStreamCaptureModeThreadLocal
+        _chk(lib().hipStreamBeginCapture(ctypes.c_void_p(self.stream.handle), ctypes.c_int(1)),
+             "hipStreamBeginCapture")
+
+    def end(self):
+        g = ctypes.c_void_p()
+        _chk(lib().hipStreamEndCapture(ctypes.c_void_p(self.stream.handle), ctypes.byref(g)),
+             "hipStreamEndCapture")
+        self.graph = g.value
+        e = ctypes.c_void_p()
+        _chk(lib().hipGraphInstantiate(ctypes.byref(e), ctypes.c_void_p(self.graph), None, None,
+                                       ctypes.c_size_t(0)), "hipGraphInstantiate")
+        self.exec = e.value
+
+    def launch(self):
+        _chk(lib().hipGraphLaunch(ctypes.c_void_p(self.exec), ctypes.c_void_p(self.stream.handle)),
+             "hipGraphLaunch")
+
+    def destroy(self):
+        if self.exec:
+            lib().hipGraphExecDestroy(ctypes.c_void_p(self.exec))
+            self.exec = None
+        if self.graph:
+            lib().hipGraphDestroy(ctypes.c_void_p(self.graph))
+            self.graph = None

@@ -81,6 +81,16 @@ class Batch:
         check(_lib.load().mdas_batch_run(self._h, t, stream.handle if stream else None,
                                          1 if use_graph else 0))
 
+    def integrate(self, pixels, widths, heights, use_graph=True, stream=None):
+        """Reconstruct image i into device float4 buffer pixels[i] (pointers or
+        DeviceBuffers) of widths[i] x heights[i]; stream-ordered."""
+        n = len(pixels)
+        ptrs = (ctypes.c_void_p * n)(*[p.ptr if hasattr(p, "ptr") else p for p in pixels])
+        w = (ctypes.c_int * n)(*[int(x) for x in widths])
+        h = (ctypes.c_int * n)(*[int(x) for x in heights])
+        check(_lib.load().mdas_batch_integrate(self._h, ptrs, w, h, stream.handle if stream else None,
+                                               1 if use_graph else 0))
+
     def autotune(self, targets, stream=None):
         """Launch shape (workgroups per launch) of the fastest of the
         candidate runs; the batch keeps it."""

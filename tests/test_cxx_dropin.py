@@ -1,5 +1,8 @@
-"""The C++ header mirror of KDGrid compiles like the reference app's code
-(CPU) and runs the reference's loop on the GPU."""
+"""The C++ header mirror of KDGrid takes the reference app's KDGrid lines
+unchanged (tests/cxx/drop_in.cpp: initMdas, samplingPassMdas, integrateMdas
+and the batch loop of optixMotionBlur.cpp, with its std::ofstream log and
+float3* sample values).  On the CPU it compiles and links; on the GPU it runs
+the app's loop and writes the reference's log lines to the app's stream."""
 import os
 import subprocess
 
@@ -14,7 +17,8 @@ def _build(out):
            "-I", "/opt/rocm/include", SRC, "-o", out, "-L", os.path.join(ROOT, "pmdas_amd"),
            "-lmdas", "-L", "/opt/rocm/lib", "-lamdhip64",
            "-Wl,-rpath," + os.path.join(ROOT, "pmdas_amd"), "-Wl,-rpath,/opt/rocm/lib"]
-    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
 
 
 def test_drop_in_compiles_and_links(tmp_path):
@@ -22,11 +26,37 @@ def test_drop_in_compiles_and_links(tmp_path):
     assert os.path.exists(tmp_path / "drop_in")
 
 
+def _log(path):
+    lines = open(path).read().split("\n")
+    keys = [(lines[i], lines[i + 1]) for i in range(0, len(lines) - 1, 2)]
+    return keys
+
+
 @pytest.mark.gpu
 def test_drop_in_runs(tmp_path):
     exe = str(tmp_path / "drop_in")
     _build(exe)
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    out = str(tmp_path / "frame.png")
+    r = subprocess.run([exe, out], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
-    assert "valid 1" in r.stdout
+    assert "valid 1 values 1" in r.stdout
+    passes = int(r.stdout.split("passes ")[1].split()[0])
+    samples = int(r.stdout.split("samples ")[1].split()[0])
+    # the app's stream: its own WIDTH / HEIGHT, then the library's lines in the
+    # order the reference writes them (kdgrid.cu:670-671, 713, 769-770, 816-817)
+    kv = _log(out + ".log")
+    keys = [k for k, _ in kv]
+    want = (["WIDTH", "HEIGHT", "INITIAL SAMPLES", "INITIAL SAMPLING TIME"] +
+            ["COMPUTE ERRORS TIME", "SAMPLES", "ADAPTIVE SAMPLING TIME"] * passes +
+            ["TOTAL SAMPLES", "CONSTRUCT TIME", "INTEGRATE TIME"])
+    assert keys == want
+    d = dict(kv)
+    assert d["WIDTH"] == "160" and d["HEIGHT"] == "90"
+    assert int(d["TOTAL SAMPLES"]) == samples  # an integer, as the reference prints it
+    assert int(kv[2][1]) == 4 << (5 + 6)  # INITIAL SAMPLES: 4 * 2^(b D + 2 e)
+    new = [int(v) for k, v in kv if k == "SAMPLES"]
+    assert sum(new) == samples - (4 << 11)
+    for k, v in kv:
+        if k.endswith("TIME"):
+            assert float(v) >= 0.0
