@@ -582,7 +582,9 @@ def main():
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 4),
                         "GBps": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)}
                     for k, v in sorted(res["kernels"].items(), key=lambda kv: -kv[1]["ms"])},
-        "device": dev_rec.get("Marketing Name") or hip.device_name() or "unknown",
+        "device": (dev_rec.get("Marketing Name") or hip.device_name() or
+                   f"{dev_rec.get('Name', 'unknown')}, {dev_rec.get('Compute Unit', '?')} CUs, "
+                   f"{dev_rec.get('Max Clock Freq. (MHz)', '?')} MHz peak engine clock"),
         "device_record": dev_rec,
     }
     if not a.no_secondary:

@@ -41,8 +41,8 @@ def test_drop_in_runs(tmp_path):
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     assert "valid 1 values 1" in r.stdout
-    passes = int(r.stdout.split("passes ")[1].split()[0])
-    samples = int(r.stdout.split("samples ")[1].split()[0])
+    summary = [ln for ln in r.stdout.splitlines() if ln.startswith("passes ")][0].split()
+    passes, samples = int(summary[1]), int(summary[3])
     # the app's stream: its own WIDTH / HEIGHT, then the library's lines in the
     # order the reference writes them (kdgrid.cu:670-671, 713, 769-770, 816-817)
     kv = _log(out + ".log")
