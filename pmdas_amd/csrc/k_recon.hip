@@ -72,7 +72,7 @@ __device__ __forceinline__ int tiles_of(int x0, int x1, int y0, int y1, const Re
 // vec_math.h:492-496); the pixel span and the xy area are recomputed from q0
 // by whoever needs them, with the same operations.
 template <int D>
-__global__ __launch_bounds__(256) void k_recon_prep(DevGrid a, ReconGeo G) {
+__global__ __launch_bounds__(256) void k_recon_prep(DevGrid a, ReconGeo G, float4* rrec) {
     __shared__ int red[4];
     const int L = a.cnt->n_leaves;
     int beg, end;
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(256) void k_recon_prep(DevGrid a, ReconGeo G) {
         int x0, x1, y0, y1;
         record_span(q0, G, x0, x1, y0, y1);
         c += tiles_of(x0, x1, y0, y1, G);
-        float4* r = a.rrec + (size_t)i * 2;
+        float4* r = rrec + (size_t)i * 2;
         r[0] = q0;
         r[1] = make_float4(sx * inv, sy * inv, sz * inv, vol);
     }
@@ -158,12 +158,11 @@ __global__ __launch_bounds__(1024) void k_scan_keys(DevGrid a, int G, long long 
     }
 }
 
-__global__ __launch_bounds__(256) void k_recon_emit(DevGrid a, ReconGeo G,
-                                                    unsigned long long* keys, long long cap_keys) {
+__global__ __launch_bounds__(256) void k_recon_emit(DevGrid a, ReconGeo G, const float4* rrec,
+                                                    unsigned long long* keys) {
     __shared__ int lds[4];
     const int L = a.cnt->n_leaves;
     if (a.cnt->recon_overflow != 0) return;  // the keys do not fit the buffer
-    (void)cap_keys;
     int beg, end;
     chunk_range(L, gridDim.x, blockIdx.x, beg, end);
     int base = a.part_off[blockIdx.x].x;
@@ -171,7 +170,7 @@ __global__ __launch_bounds__(256) void k_recon_emit(DevGrid a, ReconGeo G,
         const int i = t0 + threadIdx.x;
         int x0 = 0, x1 = 0, y0 = 0, y1 = 0, c = 0;
         if (i < end) {
-            record_span(a.rrec[(size_t)i * 2], G, x0, x1, y0, y1);
+            record_span(rrec[(size_t)i * 2], G, x0, x1, y0, y1);
             c = tiles_of(x0, x1, y0, y1, G);
         }
         int total;
@@ -362,27 +361,27 @@ static void scan_u32(unsigned* data, int n, unsigned* sums, hipStream_t st) {
 }
 
 template <int BITS>
-static void sort_pass(Grid& g, const unsigned long long* a, unsigned long long* b, int sh,
-                      hipStream_t st) {
+static void sort_pass(Grid& g, unsigned* rs_hist, const unsigned long long* a, unsigned long long* b,
+                      int sh, hipStream_t st) {
     const int* n_ptr = &g.dev.cnt->n_keys;
     const int nh = (1 << BITS) * kSortBlocks;
     kbeg(g, st, "k_rs_hist");
-    hipLaunchKernelGGL(k_rs_hist<BITS>, dim3(kSortBlocks), dim3(256), 0, st, a, n_ptr, sh, g.rs_hist);
+    hipLaunchKernelGGL(k_rs_hist<BITS>, dim3(kSortBlocks), dim3(256), 0, st, a, n_ptr, sh, rs_hist);
     kend(g, st);
     kbeg(g, st, "k_rs_scan");
-    scan_u32(g.rs_hist, nh, g.rs_hist + nh, st);
+    scan_u32(rs_hist, nh, rs_hist + nh, st);
     kend(g, st);
     kbeg(g, st, "k_rs_scatter");
     hipLaunchKernelGGL(k_rs_scatter<BITS>, dim3(kSortBlocks), dim3(256), 0, st, a, b, n_ptr, sh,
-                       g.rs_hist);
+                       rs_hist);
     kend(g, st);
 }
 
 // Sorts keys[0, n) by bits [lo, hi) in ceil((hi - lo) / 10) passes of equal
 // digit width; returns the buffer holding the result.
-static unsigned long long* radix_sort_keys(Grid& g, int lo, int hi, hipStream_t st) {
-    unsigned long long* a = g.keys;
-    unsigned long long* b = g.keys_alt;
+static unsigned long long* radix_sort_keys(Grid& g, ReconBufs& rb, int lo, int hi, hipStream_t st) {
+    unsigned long long* a = rb.keys;
+    unsigned long long* b = rb.keys_alt;
     g.recon_sort_passes = 0;
     const int total = hi - lo;
     if (total <= 0) return a;
@@ -390,13 +389,13 @@ static unsigned long long* radix_sort_keys(Grid& g, int lo, int hi, hipStream_t 
     const int bits = max(4, (total + passes - 1) / passes);
     for (int sh = lo; sh < hi; sh += bits) {
         switch (bits) {
-        case 4: sort_pass<4>(g, a, b, sh, st); break;
-        case 5: sort_pass<5>(g, a, b, sh, st); break;
-        case 6: sort_pass<6>(g, a, b, sh, st); break;
-        case 7: sort_pass<7>(g, a, b, sh, st); break;
-        case 8: sort_pass<8>(g, a, b, sh, st); break;
-        case 9: sort_pass<9>(g, a, b, sh, st); break;
-        default: sort_pass<10>(g, a, b, sh, st); break;
+        case 4: sort_pass<4>(g, rb.rs_hist, a, b, sh, st); break;
+        case 5: sort_pass<5>(g, rb.rs_hist, a, b, sh, st); break;
+        case 6: sort_pass<6>(g, rb.rs_hist, a, b, sh, st); break;
+        case 7: sort_pass<7>(g, rb.rs_hist, a, b, sh, st); break;
+        case 8: sort_pass<8>(g, rb.rs_hist, a, b, sh, st); break;
+        case 9: sort_pass<9>(g, rb.rs_hist, a, b, sh, st); break;
+        default: sort_pass<10>(g, rb.rs_hist, a, b, sh, st); break;
         }
         g.recon_sort_passes++;
         g.recon_sort_bits = bits;
@@ -420,7 +419,7 @@ __global__ __launch_bounds__(256) void k_tile_ranges(const unsigned long long* k
 // ---------------------------------------------------------------- gather --
 // GS = pixels per tile (16 or 64); a wave holds 64/GS tiles.
 template <int GS>
-__global__ __launch_bounds__(256) void k_gather(DevGrid a, ReconGeo G,
+__global__ __launch_bounds__(256) void k_gather(ReconGeo G, const float4* __restrict__ rrec,
                                                 const unsigned long long* __restrict__ keys,
                                                 const int* __restrict__ tstart,
                                                 const int* __restrict__ tend, float4* pixels) {
@@ -452,8 +451,8 @@ __global__ __launch_bounds__(256) void k_gather(DevGrid a, ReconGeo G,
     float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = c0;  // records of the round to stage
     if (gl < nrec_of(0)) {
         const int i = (int)(unsigned)(keys[s0 + gl] & 0xffffffffull);
-        c0 = a.rrec[(size_t)i * 2];
-        c1 = a.rrec[(size_t)i * 2 + 1];
+        c0 = rrec[(size_t)i * 2];
+        c1 = rrec[(size_t)i * 2 + 1];
     }
     unsigned long long knext = gl < nrec_of(1) ? keys[s0 + GS + gl] : 0ull;
     for (int rd = 0; rd < rounds; ++rd) {
@@ -495,8 +494,8 @@ __global__ __launch_bounds__(256) void k_gather(DevGrid a, ReconGeo G,
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (gl < nrec_of(rd + 1)) {
             const int i = (int)(unsigned)(knext & 0xffffffffull);
-            c0 = a.rrec[(size_t)i * 2];
-            c1 = a.rrec[(size_t)i * 2 + 1];
+            c0 = rrec[(size_t)i * 2];
+            c1 = rrec[(size_t)i * 2 + 1];
         }
         knext = gl < nrec_of(rd + 2) ? keys[s0 + (rd + 2) * GS + gl] : 0ull;
         // the round's records that cover this lane's pixel, then only those,
@@ -674,15 +673,15 @@ static ReconGeo make_geo(const Grid& g, int W, int H) {
     default: return hipErrorInvalidValue;              \
     }
 
-static hipError_t grow_keys(Grid& g, long long cap) {
-    if (g.keys) (void)hipFree(g.keys);
-    if (g.keys_alt) (void)hipFree(g.keys_alt);
-    g.keys = g.keys_alt = g.keys_sorted = nullptr;
-    g.keys_cap = 0;
-    hipError_t e = hipMalloc(&g.keys, sizeof(unsigned long long) * (size_t)cap);
-    if (e == hipSuccess) e = hipMalloc(&g.keys_alt, sizeof(unsigned long long) * (size_t)cap);
+static hipError_t grow_keys(ReconBufs& rb, long long cap) {
+    if (rb.keys) (void)hipFree(rb.keys);
+    if (rb.keys_alt) (void)hipFree(rb.keys_alt);
+    rb.keys = rb.keys_alt = nullptr;
+    rb.keys_cap = 0;
+    hipError_t e = hipMalloc(&rb.keys, sizeof(unsigned long long) * (size_t)cap);
+    if (e == hipSuccess) e = hipMalloc(&rb.keys_alt, sizeof(unsigned long long) * (size_t)cap);
     if (e != hipSuccess) return e;
-    g.keys_cap = cap;
+    rb.keys_cap = cap;
     return hipSuccess;
 }
 
@@ -701,47 +700,65 @@ static long long tiles_per_leaf(const Grid& g, const ReconGeo& G) {
     return axis(g.ext_x, G.W, G.sX, G.TW, G.tilesX) * axis(g.ext_y, G.H, G.sY, G.TH, G.tilesY);
 }
 
+static long long key_bound(const Grid& g, const ReconGeo& G) {
+    return tiles_per_leaf(g, G) * std::max<long long>(g.host.n_leaves, 1);
+}
+
 // Key buffers of at most this many bytes are sized from the bound up front
 // (no host round trip in the reconstruction); larger bounds (tiny initial
 // grids, whose leaves span many tiles) size them from the counted keys.
 static const long long kKeyBoundBytes = 8ll << 30;
 
-// Sizes the reconstruction buffers for the current leaf count (g.host is
-// current after every call that changes the tree) and image; allocates only
-// when they grow.  Sets g.recon_bounded when the key buffers hold the bound,
-// i.e. the reconstruction needs no host round trip.
+static long long align256(long long b) { return (b + 255) / 256 * 256; }
+
+long long recon_key_bound(const Grid& g, int W, int H) { return key_bound(g, make_geo(g, W, H)); }
+long long recon_tiles_cap(const Grid& g, int W, int H) { return 1ll << (2 * make_geo(g, W, H).tbits); }
+
+long long recon_bytes_needed(const Grid& g, int W, int H) {
+    const ReconGeo G = make_geo(g, W, H);
+    const long long kb = key_bound(g, G);
+    if (kb > 0x7fffffffLL) return -1;
+    const long long L = std::max<long long>(g.host.n_leaves, 1);
+    return align256(sizeof(unsigned) * kSortHistWords) + 2 * align256(sizeof(int) << (2 * G.tbits)) +
+           align256(sizeof(float4) * 2 * L) + 2 * align256(sizeof(unsigned long long) * kb);
+}
+
+// Sizes the grid's reconstruction buffers for the current leaf count (g.host
+// is current after every call that changes the tree) and image; allocates
+// only when they grow.  Sets g.recon_bounded when the key buffers hold the
+// bound, i.e. the reconstruction needs no host round trip.
 hipError_t recon_reserve(Grid& g, int W, int H) {
     ReconGeo G = make_geo(g, W, H);
+    ReconBufs& rb = g.rb;
     const long long L = g.host.n_leaves;
-    if (L > g.rrec_cap) {
-        if (g.dev.rrec) (void)hipFree(g.dev.rrec);
-        g.dev.rrec = nullptr;
-        g.rrec_cap = 0;
+    if (L > rb.rrec_cap) {
+        if (rb.rrec) (void)hipFree(rb.rrec);
+        rb.rrec = nullptr;
+        rb.rrec_cap = 0;
         const long long cap = std::min<long long>((long long)g.cap_l, L + L / 4 + 1024);
-        hipError_t e = hipMalloc(&g.dev.rrec, sizeof(float4) * 2 * (size_t)cap);
+        hipError_t e = hipMalloc(&rb.rrec, sizeof(float4) * 2 * (size_t)cap);
         if (e != hipSuccess) return e;
-        g.rrec_cap = cap;
+        rb.rrec_cap = cap;
     }
-    const long long kb = tiles_per_leaf(g, G) * std::max<long long>(L, 1);
+    const long long kb = key_bound(g, G);
     g.recon_bounded = kb <= 0x7fffffffLL && kb * 16 <= kKeyBoundBytes;
-    if (g.recon_bounded && kb > g.keys_cap) {
-        hipError_t e = grow_keys(g, kb);
+    if (g.recon_bounded && kb > rb.keys_cap) {
+        hipError_t e = grow_keys(rb, kb);
         if (e != hipSuccess) return e;
     }
     const long long tiles = 1ll << (2 * G.tbits);
-    if (tiles > g.tiles_cap) {
-        if (g.tile_start) (void)hipFree(g.tile_start);
-        if (g.tile_end) (void)hipFree(g.tile_end);
-        g.tile_start = g.tile_end = nullptr;
-        g.tiles_cap = 0;
-        hipError_t e = hipMalloc(&g.tile_start, sizeof(int) * tiles);
-        if (e == hipSuccess) e = hipMalloc(&g.tile_end, sizeof(int) * tiles);
+    if (tiles > rb.tiles_cap) {
+        if (rb.tile_start) (void)hipFree(rb.tile_start);
+        if (rb.tile_end) (void)hipFree(rb.tile_end);
+        rb.tile_start = rb.tile_end = nullptr;
+        rb.tiles_cap = 0;
+        hipError_t e = hipMalloc(&rb.tile_start, sizeof(int) * tiles);
+        if (e == hipSuccess) e = hipMalloc(&rb.tile_end, sizeof(int) * tiles);
         if (e != hipSuccess) return e;
-        g.tiles_cap = tiles;
+        rb.tiles_cap = tiles;
     }
-    if (!g.rs_hist) {
-        // histograms plus the block sums of their scan
-        hipError_t e = hipMalloc(&g.rs_hist, sizeof(unsigned) * ((1 << kSortMaxBits) * kSortBlocks + 1024));
+    if (!rb.rs_hist) {
+        hipError_t e = hipMalloc(&rb.rs_hist, sizeof(unsigned) * kSortHistWords);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -757,50 +774,47 @@ hipError_t recon_prepare(Grid& g, int W, int H, hipStream_t st) {
     g.recon_pixels = (long long)W * H;
     kbeg(g, st, "k_recon_prep");
     MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_recon_prep<DD>, dim3(g.chunk_blocks), dim3(256), 0, st,
-                                            g.dev, G));
+                                            g.dev, G, g.rb.rrec));
+    kend(g, st);
+    kbeg(g, st, "k_scan_keys");
+    hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, st, g.dev, g.chunk_blocks,
+                       g.recon_bounded ? g.rb.keys_cap : (long long)0x7fffffff);
     kend(g, st);
     if (!g.recon_bounded) {
-        kbeg(g, st, "k_scan_keys");
-        hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, st, g.dev, g.chunk_blocks,
-                           (long long)0x7fffffff);
-        kend(g, st);
         int nk = 0;
         e = hipMemcpyAsync(&nk, &g.dev.cnt->n_keys, sizeof(int), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) return e;
-        if (nk > g.keys_cap) {
-            e = grow_keys(g, (long long)nk + nk / 4 + 1024);
+        if (nk > g.rb.keys_cap) {
+            e = grow_keys(g.rb, (long long)nk + nk / 4 + 1024);
             if (e != hipSuccess) return e;
         }
-    } else {
-        kbeg(g, st, "k_scan_keys");
-        hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, st, g.dev, g.chunk_blocks, g.keys_cap);
-        kend(g, st);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_integrate_gather(Grid& g, float* pixels, int W, int H, hipStream_t st,
-                                   hipEvent_t ev_sort0, hipEvent_t ev_sort1) {
+// Keys in leaf order, their sort, the tiles' key ranges and the gather, on
+// the buffers of rb (records already written by k_recon_prep).
+static hipError_t recon_finish(Grid& g, ReconBufs& rb, float* pixels, int W, int H, hipStream_t st,
+                               hipEvent_t ev_sort0, hipEvent_t ev_sort1) {
     ReconGeo G = make_geo(g, W, H);
     kbeg(g, st, "k_recon_emit");
-    hipLaunchKernelGGL(k_recon_emit, dim3(g.chunk_blocks), dim3(256), 0, st, g.dev, G, g.keys,
-                       g.keys_cap);
+    hipLaunchKernelGGL(k_recon_emit, dim3(g.chunk_blocks), dim3(256), 0, st, g.dev, G, rb.rrec, rb.keys);
     kend(g, st);
     if (ev_sort0) (void)hipEventRecord(ev_sort0, st);
-    // the sorted keys end in g.keys or g.keys_alt by the parity of the pass
+    // the sorted keys end in keys or keys_alt by the parity of the pass
     // count; the buffers keep their roles, so a captured graph stays valid
-    unsigned long long* sorted = radix_sort_keys(g, 32, 32 + 2 * G.tbits, st);
+    unsigned long long* sorted = radix_sort_keys(g, rb, 32, 32 + 2 * G.tbits, st);
     g.keys_sorted = sorted;
     if (ev_sort1) (void)hipEventRecord(ev_sort1, st);
     const long long tiles = 1ll << (2 * G.tbits);
     kbeg(g, st, "k_tile_ranges");
-    launch_fill_i32(g.tile_start, 0, tiles, st);
-    launch_fill_i32(g.tile_end, 0, tiles, st);
-    int blocks = (int)std::min<long long>((g.keys_cap + 255) / 256, 8192);  // grid-stride over
+    launch_fill_i32(rb.tile_start, 0, tiles, st);
+    launch_fill_i32(rb.tile_end, 0, tiles, st);
+    int blocks = (int)std::min<long long>((rb.keys_cap + 255) / 256, 8192);  // grid-stride over
     if (blocks < 1) blocks = 1;                                              // the device count
     hipLaunchKernelGGL(k_tile_ranges, dim3(blocks), dim3(256), 0, st, sorted, &g.dev.cnt->n_keys,
-                       g.tile_start, g.tile_end);
+                       rb.tile_start, rb.tile_end);
     kend(g, st);
     const int ntiles = G.tilesX * G.tilesY;
     const int gs = G.TW * G.TH;
@@ -808,14 +822,28 @@ hipError_t launch_integrate_gather(Grid& g, float* pixels, int W, int H, hipStre
     kbeg(g, st, "k_gather");
     if (gs == 64)
         hipLaunchKernelGGL(k_gather<64>, dim3((ntiles + tiles_per_block - 1) / tiles_per_block),
-                           dim3(256), 0, st, g.dev, G, sorted, g.tile_start, g.tile_end,
+                           dim3(256), 0, st, G, rb.rrec, sorted, rb.tile_start, rb.tile_end,
                            (float4*)pixels);
     else
         hipLaunchKernelGGL(k_gather<16>, dim3((ntiles + tiles_per_block - 1) / tiles_per_block),
-                           dim3(256), 0, st, g.dev, G, sorted, g.tile_start, g.tile_end,
+                           dim3(256), 0, st, G, rb.rrec, sorted, rb.tile_start, rb.tile_end,
                            (float4*)pixels);
     kend(g, st);
     return hipGetLastError();
+}
+
+hipError_t launch_integrate_gather(Grid& g, float* pixels, int W, int H, hipStream_t st,
+                                   hipEvent_t ev_sort0, hipEvent_t ev_sort1) {
+    return recon_finish(g, g.rb, pixels, W, H, st, ev_sort0, ev_sort1);
+}
+
+hipError_t recon_on_buffers(Grid& g, ReconBufs& rb, float* pixels, int W, int H, hipStream_t st) {
+    ReconGeo G = make_geo(g, W, H);
+    g.recon_pixels = (long long)W * H;
+    MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_recon_prep<DD>, dim3(g.chunk_blocks), dim3(256), 0, st,
+                                            g.dev, G, rb.rrec));
+    hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, st, g.dev, g.chunk_blocks, rb.keys_cap);
+    return recon_finish(g, rb, pixels, W, H, st, nullptr, nullptr);
 }
 
 hipError_t launch_integrate_scatter(Grid& g, float* pixels, int W, int H, hipStream_t st) {

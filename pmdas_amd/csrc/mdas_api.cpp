@@ -58,18 +58,19 @@ void lcg_jump(unsigned k, unsigned& A, unsigned& C, unsigned& Ainv) {
 
 void free_grid(Grid& g) {
     mdas::DevGrid& d = g.dev;
-    void* ptrs[] = {d.coords,   d.values, d.leaf,     d.err,  d.seed,       d.flags, d.pts,
-                    d.split_list, d.sdat, d.sext, d.part, d.part_off, d.epart, d.rrec,      d.cnt,
-                    d.pstats,   g.keys,   g.keys_alt, g.tile_start, g.tile_end, g.rs_hist,
-                    g.d_scalar, g.learn.feat, g.learn.perr, g.learn.rows};
+    mdas::ReconBufs& rb = g.rb;
+    void* ptrs[] = {d.coords,   d.values,  d.leaf,        d.err,        d.seed,     d.flags,
+                    d.pts,      d.split_list, d.sdat,     d.sext,       d.part,     d.part_off,
+                    d.epart,    d.cnt,     d.pstats,      rb.rrec,      rb.keys,    rb.keys_alt,
+                    rb.tile_start, rb.tile_end, rb.rs_hist, g.d_scalar, g.learn.feat, g.learn.perr,
+                    g.learn.rows};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : g.evpool) (void)hipEventDestroy(e);
     g.evpool.clear();
     d = mdas::DevGrid{};
-    g.keys = g.keys_alt = g.keys_sorted = nullptr;
-    g.tile_start = g.tile_end = nullptr;
-    g.rs_hist = nullptr;
+    rb = mdas::ReconBufs{};
+    g.keys_sorted = nullptr;
     g.d_scalar = nullptr;
     g.learn = mdas::LearnDev{};
     g.learn_mode = 0;

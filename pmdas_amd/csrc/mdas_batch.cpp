@@ -304,10 +304,10 @@ int mdas_batch_integrate(mdas_batch_handle b, float* const* pixels, const int* w
         const unsigned long long k[] = {
             (unsigned long long)(uintptr_t)pixels[i], (unsigned long long)widths[i],
             (unsigned long long)heights[i], (unsigned long long)g.tile_w, (unsigned long long)g.tile_h,
-            (unsigned long long)g.chunk_blocks, (unsigned long long)(uintptr_t)g.dev.rrec,
-            (unsigned long long)(uintptr_t)g.keys, (unsigned long long)(uintptr_t)g.keys_alt,
-            (unsigned long long)g.keys_cap, (unsigned long long)(uintptr_t)g.tile_start,
-            (unsigned long long)(uintptr_t)g.tile_end, (unsigned long long)(uintptr_t)g.rs_hist};
+            (unsigned long long)g.chunk_blocks, (unsigned long long)(uintptr_t)g.rb.rrec,
+            (unsigned long long)(uintptr_t)g.rb.keys, (unsigned long long)(uintptr_t)g.rb.keys_alt,
+            (unsigned long long)g.rb.keys_cap, (unsigned long long)(uintptr_t)g.rb.tile_start,
+            (unsigned long long)(uintptr_t)g.rb.tile_end, (unsigned long long)(uintptr_t)g.rb.rs_hist};
         key.insert(key.end(), k, k + sizeof(k) / sizeof(k[0]));
     }
     if (!b->recon_exec || key != b->recon_key) {

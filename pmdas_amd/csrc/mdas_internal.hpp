@@ -82,7 +82,6 @@ struct DevGrid {
     int4* part;            // [kChunkBlocks] per-workgroup (selected, split, samples, carry)
     int2* part_off;        // [kChunkBlocks] exclusive prefix of (selected, split)
     uint4* epart;          // [kErrBlocks] per-workgroup (max error bits, stale, values)
-    float4* rrec;          // [L * 2]      reconstruction record per leaf
     unsigned char* rcnt;   // [cap_l]      reconstruction tiles per leaf
     DevCounters* cnt;
     PassStat* pstats;
@@ -95,6 +94,21 @@ struct DevGrid {
     int b, e;
     float alpha, sX, sY;
 };
+
+// Buffers of one reconstruction: a 32-byte record per leaf, the keys (two
+// buffers for the sort), every tile's key range and the sort's histograms.
+struct ReconBufs {
+    float4* rrec = nullptr;                 // [rrec_cap * 2]
+    long long rrec_cap = 0;                 // leaves the records can hold
+    unsigned long long* keys = nullptr;     // [keys_cap]
+    unsigned long long* keys_alt = nullptr; // [keys_cap]
+    long long keys_cap = 0;
+    int* tile_start = nullptr;              // [tiles_cap]
+    int* tile_end = nullptr;                // [tiles_cap]
+    long long tiles_cap = 0;
+    unsigned* rs_hist = nullptr;            // [kSortHistWords]
+};
+static const long long kSortHistWords = 1024ll * 1024 + 1024;  // 10-bit digits x 1024 blocks + sums
 
 // One MLP of the learned proposal (16 -> 64 -> 64 -> 1).  p is the FP32
 // parameter blob (layout in mlp_device.hpp); the others are the bf16 forms
@@ -141,17 +155,10 @@ struct Grid {
     void (*log_fn)(void* user, const char* key, double value, int is_count) = nullptr;
     void* log_user = nullptr;
     bool logging() const { return !log_path.empty() || log_fn != nullptr; }
-    // reconstruction scratch
-    unsigned long long* keys = nullptr;
-    unsigned long long* keys_alt = nullptr;
-    unsigned long long* keys_sorted = nullptr;  // keys or keys_alt: the last sort's result
-    long long keys_cap = 0;
-    int* tile_start = nullptr;
-    int* tile_end = nullptr;
-    long long tiles_cap = 0;
-    unsigned* rs_hist = nullptr;  // radix histogram [1024 * kSortBlocks] + scan sums
+    // reconstruction buffers (allocated at first use, grown with the tree)
+    ReconBufs rb;
+    unsigned long long* keys_sorted = nullptr;  // rb.keys or rb.keys_alt: the last sort's result
     int tile_w = 8, tile_h = 8;  // reconstruction tile (mdas_set_recon_tile / autotune)
-    long long rrec_cap = 0;       // leaves the reconstruction records can hold
     bool recon_bounded = false;   // the key buffers hold the bound of the last reservation
     // largest leaf extent along x and y (sample-space units): the initial cell's,
     // which splits only shrink, or the largest of an uploaded state's boxes
@@ -241,6 +248,15 @@ hipError_t launch_qmc_image(const void* dparams, int W, int H, int spp, int firs
 // Reconstruction (k_recon.hip)
 hipError_t launch_integrate_gather(Grid& g, float* pixels, int W, int H, hipStream_t st,
                                    hipEvent_t ev_sort0, hipEvent_t ev_sort1);
+// The whole reconstruction on the given buffers (sized by the caller for the
+// bound; no allocation, no host round trip); sets g.keys_sorted.
+hipError_t recon_on_buffers(Grid& g, ReconBufs& rb, float* pixels, int W, int H, hipStream_t st);
+// Bytes of buffers a reconstruction of the current tree into a W x H image
+// needs with its key bound (-1 if the bound is not finite in 32-bit keys).
+long long recon_bytes_needed(const Grid& g, int W, int H);
+// Its parts: the key bound and the tile-range entries.
+long long recon_key_bound(const Grid& g, int W, int H);
+long long recon_tiles_cap(const Grid& g, int W, int H);
 hipError_t launch_integrate_scatter(Grid& g, float* pixels, int W, int H, hipStream_t st);
 hipError_t launch_bytes(const float* pixels, unsigned char* out, long long n, hipStream_t st);
 
