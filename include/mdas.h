@@ -86,6 +86,18 @@ int mdas_sampling_pass(mdas_handle h, void* stream);
  * for trees whose key bound exceeds 8 GiB (initial cells spanning many
  * pixel tiles), where the key count is read back first. */
 int mdas_integrate(mdas_handle h, float* pixels, int width, int height, void* stream);
+/* No counterpart in the reference: mdas_integrate for callers that cannot
+ * allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same kernels and pixels as
+ * mdas_integrate, on a scratch array of MDAS_NOALLOC_SCRATCH_BYTES that the
+ * library holds on each GPU it runs on until the process ends; the call makes
+ * no allocation and no host round trip.  A tree and image whose
+ * reconstruction buffers (leaf records, keys at their bound, tile ranges,
+ * sort histograms) exceed the array return MDAS_ERR_CAPACITY before anything
+ * runs.  Every call on one GPU shares that GPU's array: calls that overlap
+ * on one GPU (other streams, other threads) race; serialise them. */
+#define MDAS_NOALLOC_SCRATCH_BYTES (256u << 20)
+int mdas_integrate_noalloc(mdas_handle h, float* pixels, int width, int height, void* stream);
 /* Reference-style reconstruction with float atomics (kdgrid.cu:483-582);
  * same result up to summation order.  Kept for comparison. */
 int mdas_integrate_scatter(mdas_handle h, float* pixels, int width, int height, void* stream);

@@ -130,6 +130,7 @@ SIGNATURES = {
     "mdas_get_passes": (_I, [_H]),
     "mdas_set_timing": (_I, [_H, _I]),
     "mdas_set_log_sink": (_I, [_H, _P, _P]),
+    "mdas_integrate_noalloc": (_I, [_H, _P, _I, _I, _P]),
     "mdas_get_phase_times": (_I, [_H, ctypes.POINTER(MdasPhaseTimes)]),
     "mdas_get_pass_stats": (_I, [_H, _P, _I]),
     "mdas_kernel_stats": (_I, [_H, _I, ctypes.c_char_p, _I, ctypes.POINTER(_I),
