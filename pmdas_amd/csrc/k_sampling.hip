@@ -30,35 +30,46 @@ __device__ __forceinline__ void initial_body(const DevGrid& a, int L0, int blk, 
             float off[D], ext[D];
             morton_cell<D>((unsigned)i, a.b, a.e, a.sX, a.sY, off, ext);
             unsigned s = tea4((unsigned)i, 0u);
+            float pts[4][D];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 float r[D];
 #pragma unroll
                 for (int d = 0; d < D; ++d) r[d] = rnd(s);
 #pragma unroll
-                for (int d = 0; d < D; ++d) stage[threadIdx.x * 4 * D + j * D + d] = off[d] + r[d] * ext[d];
+                for (int d = 0; d < D; ++d) {
+                    pts[j][d] = off[d] + r[d] * ext[d];
+                    stage[threadIdx.x * 4 * D + j * D + d] = pts[j][d];
+                }
             }
             float mx[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) mx[d] = off[d] + ext[d];
-            store_leaf<D>(a, i, make_int4(4 * i, 4 * i + 1, 4 * i + 2, 4 * i + 3), off, mx);
+            // the leaf's line (slots, box, first points) and its overflow row
+            store_line<D>(a, i, make_int4(4 * i, 4 * i + 1, 4 * i + 2, 4 * i + 3), off, mx, pts, 4);
+            if constexpr (line_pts(D) < 4) {
+                constexpr int P = line_pts(D), XS = xpts_stride(D);
+                float x[XS];
+#pragma unroll
+                for (int k = 0; k < XS; ++k) x[k] = 0.0f;
+#pragma unroll
+                for (int q = P; q < 4; ++q)
+#pragma unroll
+                    for (int d = 0; d < D; ++d) x[(q - P) * D + d] = pts[q][d];
+                float4* xr = reinterpret_cast<float4*>(a.xpts + (size_t)i * XS);
+#pragma unroll
+                for (int k = 0; k < XS / 4; ++k) xr[k] = make_float4(x[4 * k], x[4 * k + 1], x[4 * k + 2], x[4 * k + 3]);
+            }
             a.seed[i] = s;
             a.err[i] = -1.0f;
             a.flags[i] = 4;
         }
         __syncthreads();
-        // leaf i's samples are 4i..4i+3, so the per-leaf point copies of the
-        // workgroup's leaves hold the same floats, one padded row per leaf
+        // leaf i's samples are 4i..4i+3: the workgroup's coordinates are one
+        // contiguous range, stored from LDS in full lines
         const int nl = min(256, L0 - base);
         float* out = a.coords + (size_t)base * 4 * D;
-        float* outp = a.pts + (size_t)base * pts_stride(D);
         for (int k = threadIdx.x; k < nl * 4 * D; k += 256) out[k] = stage[k];
-        // whole padded rows, so that the stores cover full lines
-        constexpr int PS = pts_stride(D);
-        for (int k = threadIdx.x; k < nl * PS; k += 256) {
-            const int l = k / PS, r = k - l * PS;
-            outp[k] = r < 4 * D ? stage[l * 4 * D + r] : 0.0f;
-        }
         __syncthreads();
     }
 }
@@ -457,35 +468,46 @@ __global__ __launch_bounds__(1024) void k_scan_counts_b(const DevGrid* grids, in
 template <int D>
 __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m, unsigned cnt,
                                            int L, unsigned s) {
-    constexpr int Q = leaf_quads(D);
-    int4 id4;
+    constexpr int P = line_pts(D);
+    // the leaf's line: slots, box and its first points, one 128-byte read
+    float f[kLineFloats];
+    load_line(a, i, f);
+    const int ids[4] = {__float_as_int(f[0]), __float_as_int(f[1]), __float_as_int(f[2]),
+                        __float_as_int(f[3])};
     float mn[D], mx[D], diag[D];
-    load_leaf<D>(a, i, id4, mn, mx);
 #pragma unroll
-    for (int d = 0; d < D; ++d) diag[d] = mx[d] - mn[d];
-    const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
-    // the leaf's points from its contiguous copy (slot order) instead of
-    // gathers at the sample indices
-    // (the row is read whole, as D float4s; the floats of empty slots are
-    // replaced, never used in arithmetic)
-    float p[4][D];
-    float* lp = a.pts + (size_t)i * pts_stride(D);
-    float row[4 * D];
-    {
-        const float4* r4 = reinterpret_cast<const float4*>(lp);
-#pragma unroll
-        for (int v = 0; v < D; ++v) {
-            const float4 t = r4[v];
-            row[4 * v] = t.x;
-            row[4 * v + 1] = t.y;
-            row[4 * v + 2] = t.z;
-            row[4 * v + 3] = t.w;
-        }
+    for (int d = 0; d < D; ++d) {
+        mn[d] = f[4 + d];
+        mx[d] = f[4 + D + d];
+        diag[d] = mx[d] - mn[d];
     }
+    // the points by slot (the floats of empty slots are replaced, never used
+    // in arithmetic); those past the line only when the leaf holds them
+    float p[4][D];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < P; ++q)
 #pragma unroll
-        for (int d = 0; d < D; ++d) p[q][d] = ids[q] >= 0 ? row[q * D + d] : 0.0f;
+        for (int d = 0; d < D; ++d) p[q][d] = ids[q] >= 0 ? f[line_pt(D, q) + d] : 0.0f;
+    if constexpr (P < 4) {
+        constexpr int XQ = ((4 - P) * D + 3) / 4;  // float4s of the overflow row in use
+        float x[4 * XQ];
+#pragma unroll
+        for (int t = 0; t < 4 * XQ; ++t) x[t] = 0.0f;
+        if (cnt > (unsigned)P) {
+            const float4* xr = reinterpret_cast<const float4*>(a.xpts + (size_t)i * xpts_stride(D));
+#pragma unroll
+            for (int t = 0; t < XQ; ++t) {
+                const float4 v = xr[t];
+                x[4 * t] = v.x;
+                x[4 * t + 1] = v.y;
+                x[4 * t + 2] = v.z;
+                x[4 * t + 3] = v.w;
+            }
+        }
+#pragma unroll
+        for (int q = P; q < 4; ++q)
+#pragma unroll
+            for (int d = 0; d < D; ++d) p[q][d] = ids[q] >= 0 ? x[(q - P) * D + d] : 0.0f;
     }
     float maxDist = -1.0f;
     float best[D];
@@ -525,15 +547,13 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
     for (int d = 0; d < D; ++d) out[d] = best[d];
 
     if (cnt < 4u) {
-        // kdgrid.cu:393-396: insert; the error is kept
-        int4 n4 = id4;
-        if (cnt == 0u) n4.x = k;
-        else if (cnt == 1u) n4.y = k;
-        else if (cnt == 2u) n4.z = k;
-        else n4.w = k;
-        store_slots(a, i, Q, n4);
+        // kdgrid.cu:393-396: insert into the first free slot; the error is kept
+        reinterpret_cast<int*>(a.leaf)[(size_t)i * kLineFloats + cnt] = k;
+        float* dst = cnt < (unsigned)P
+                         ? reinterpret_cast<float*>(a.leaf) + (size_t)i * kLineFloats + line_pt(D, 0) + cnt * D
+                         : a.xpts + (size_t)i * xpts_stride(D) + (cnt - P) * D;
 #pragma unroll
-        for (int d = 0; d < D; ++d) lp[cnt * D + d] = best[d];
+        for (int d = 0; d < D; ++d) dst[d] = best[d];
     } else {
         // kdgrid.cu:399-471: split.  LargestAxis scans from the last axis down
         // with a strict '<' (aabb.h:34-45).
@@ -589,8 +609,29 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
             cmn[d] = d == axis ? pos : mn[d];
             cmx[d] = d == axis ? pos : mx[d];
         }
-        store_leaf<D>(a, i, make_int4(v[0], v[1], E, E), mn, cmx);
-        store_leaf<D>(a, m, make_int4(v[2], v[3], v[4], E), cmn, mx);
+        // the points by sorted position: 0, 1 -> left child, 2..4 -> right
+        float sp[5][D];
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                float x = best[d];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (perm[j] == q) x = p[q][d];
+                sp[j][d] = x;
+            }
+        // the left child in the parent's line (slots, box, two points: whole
+        // float4s, the rest stays and is marked empty), the right child's
+        // whole line, and its third point in its overflow row when the line
+        // holds two (D = 6)
+        store_line_head2<D>(a, i, make_int4(v[0], v[1], E, E), mn, cmx, sp[0], sp[1]);
+        store_line<D>(a, m, make_int4(v[2], v[3], v[4], E), cmn, mx, &sp[2], 3);
+        if constexpr (P < 3) {
+            float* xr = a.xpts + (size_t)m * xpts_stride(D);
+#pragma unroll
+            for (int d = 0; d < D; ++d) xr[d] = sp[4][d];
+        }
         // what the next error pass needs of both children, coalesced by
         // split rank: their sample indices and box volumes
         a.sdat[2 * (m - L)] = make_int4(v[0], v[1], v[2], v[3]);
@@ -608,23 +649,6 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
                 if (d == axis) right = mx[d] - cmn[d];
             a.sext[2 * (m - L) + 1] = make_int4(__float_as_int(ex[4]), __float_as_int(ex[5]),
                                                 __float_as_int(right), axis);
-        }
-        // point copies: sorted positions 0, 1 -> left child, 2..4 -> right
-        float* rp = a.pts + (size_t)m * pts_stride(D);
-#pragma unroll
-        for (int j = 0; j < 5; ++j) {
-            float pt[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                float x = best[d];
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (perm[j] == q) x = p[q][d];
-                pt[d] = x;
-            }
-            float* dst = j < 2 ? lp + j * D : rp + (j - 2) * D;
-#pragma unroll
-            for (int d = 0; d < D; ++d) dst[d] = pt[d];
         }
         if (!a.in_loop) {  // in the loop the next error pass overwrites both
             a.err[i] = -1.0f;
@@ -750,19 +774,19 @@ __global__ __launch_bounds__(256) void k_mark_last_children_b(const DevGrid* gri
     mark_last_children_body(grids[blockIdx.y], blockIdx.x, gridDim.x);
 }
 
-// Per-leaf point copies from the sample coordinates (after a state upload).
+// The leaves' points in their lines and overflow rows from the sample
+// coordinates (after a state upload).
 template <int D>
 __global__ __launch_bounds__(256) void k_build_pts(DevGrid a, int L) {
-    constexpr int Q = leaf_quads(D);
     for (int i = blockIdx.x * 256 + threadIdx.x; i < L; i += gridDim.x * 256) {
-        const int4 id = load_slots(a, i, Q);
+        const int4 id = load_slots(a, i);
         const int ids[4] = {id.x, id.y, id.z, id.w};
-        float* lp = a.pts + (size_t)i * pts_stride(D);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             if (ids[q] >= 0 && ids[q] < a.cap_s) {
+                float* dst = point_addr<D>(a, i, q);
 #pragma unroll
-                for (int d = 0; d < D; ++d) lp[q * D + d] = a.coords[(size_t)ids[q] * D + d];
+                for (int d = 0; d < D; ++d) dst[d] = a.coords[(size_t)ids[q] * D + d];
             }
         }
     }

@@ -60,7 +60,7 @@ void free_grid(Grid& g) {
     mdas::DevGrid& d = g.dev;
     mdas::ReconBufs& rb = g.rb;
     void* ptrs[] = {d.coords,   d.values,  d.leaf,        d.err,        d.seed,     d.flags,
-                    d.pts,      d.split_list, d.sdat,     d.sext,       d.part,     d.part_off,
+                    d.xpts,     d.split_list, d.sdat,     d.sext,       d.part,     d.part_off,
                     d.epart,    d.cnt,     d.pstats,      rb.rrec,      rb.keys,    rb.keys_alt,
                     rb.tile_start, rb.tile_end, rb.rs_hist, g.d_scalar, g.learn.feat, g.learn.perr,
                     g.learn.rows};
@@ -211,8 +211,8 @@ int mdas_create(const mdas_config* cfg, mdas_handle* out) {
     const size_t cs = (size_t)g.cap_s, cl = (size_t)g.cap_l;
     if (e == hipSuccess) e = dalloc(&d.coords, cs * D);
     if (e == hipSuccess) e = dalloc(&d.values, cs * 3);
-    if (e == hipSuccess) e = dalloc(&d.leaf, cl * mdas::leaf_quads(D));
-    if (e == hipSuccess) e = dalloc(&d.pts, cl * mdas::pts_stride(D));
+    if (e == hipSuccess) e = dalloc(&d.leaf, cl * (mdas::kLineFloats / 4));
+    if (e == hipSuccess && mdas::xpts_stride(D) > 0) e = dalloc(&d.xpts, cl * mdas::xpts_stride(D));
     if (e == hipSuccess) e = dalloc(&d.err, cl);
     if (e == hipSuccess) e = dalloc(&d.seed, cl);
     if (e == hipSuccess) e = dalloc(&d.flags, cl);
@@ -461,12 +461,12 @@ int mdas_download_state(mdas_handle h, float* coords, float* values, int* idx, f
     if (values) HIP_TRY(hipMemcpy(values, g.dev.values, sizeof(float) * N * 3, hipMemcpyDeviceToHost));
     if (err) HIP_TRY(hipMemcpy(err, g.dev.err, sizeof(float) * L, hipMemcpyDeviceToHost));
     if (seed) HIP_TRY(hipMemcpy(seed, g.dev.seed, sizeof(uint32_t) * L, hipMemcpyDeviceToHost));
-    if (idx || box) {
-        const int Q = mdas::leaf_quads(D);
-        std::vector<float> rec(L * 4 * Q);
+    if (idx || box) {  // slots and box from each leaf's line
+        const size_t LF = mdas::kLineFloats;
+        std::vector<float> rec(L * LF);
         HIP_TRY(hipMemcpy(rec.data(), g.dev.leaf, sizeof(float) * rec.size(), hipMemcpyDeviceToHost));
         for (size_t i = 0; i < L; ++i) {
-            const float* r = &rec[i * 4 * Q];
+            const float* r = &rec[i * LF];
             if (idx) memcpy(idx + 4 * i, r, 16);
             if (box) memcpy(box + i * 2 * D, r + 4, sizeof(float) * 2 * D);
         }
@@ -506,14 +506,14 @@ int mdas_upload_state(mdas_handle h, int n, int l, const float* coords, const fl
         memcpy(s.data(), seed, sizeof(uint32_t) * (size_t)l);
         HIP_TRY(hipMemcpy(g.dev.seed, s.data(), sizeof(uint32_t) * s.size(), hipMemcpyHostToDevice));
     }
-    if (idx) {
-        const int Q = mdas::leaf_quads(D);
-        std::vector<float> rec((size_t)l * 4 * Q, 0.0f);
+    if (idx) {  // lines with slots and box; the points follow from the coordinates
+        const size_t LF = mdas::kLineFloats;
+        std::vector<float> rec((size_t)l * LF, 0.0f);
         std::vector<unsigned char> f((size_t)l);
         for (int i = 0; i < l; ++i) {
             int c = 0;
             for (int q = 0; q < 4; ++q) c += idx[4 * (size_t)i + q] >= 0 ? 1 : 0;
-            float* r = &rec[(size_t)i * 4 * Q];
+            float* r = &rec[(size_t)i * LF];
             memcpy(r, idx + 4 * (size_t)i, 16);
             memcpy(r + 4, box + (size_t)i * 2 * D, sizeof(float) * 2 * D);
             f[i] = (unsigned char)c;
@@ -834,24 +834,26 @@ int mdas_get_pass_stats(mdas_handle h, int* out, int max_rows) {
 
 // Algorithmic bytes of each kernel over the last run + reconstruction: the
 // bytes the algorithm must move with this layout, each counted once: FP32
-// coordinates (4*D per sample) and values (12), leaf records of LB = 48 or 64
-// bytes (slots + box, read or written whole), the per-leaf copies of the
-// leaf's points (4*D per point), 4-byte errors and LCG states, 1-byte flags,
-// 8-byte keys, 32-byte reconstruction records, float4 pixels.
+// coordinates (4*D per sample) and values (12), a leaf's slots and box
+// (SB = 16 + 8*D bytes of its line) and its points (4*D each, in the line or
+// the overflow row), 4-byte errors and LCG states, 1-byte flags, 8-byte keys,
+// 32-byte reconstruction records, float4 pixels.  Line padding is not
+// counted.
 static double kernel_bytes(const Grid& g, const std::string& k) {
     const double D = g.D;
-    const double LB = 16.0 * mdas::leaf_quads(g.D);
+    const double SB = 16.0 + 8.0 * D;
     const double L0 = (double)(1LL << (g.b * g.D + 2 * g.e));
     double b = 0.0;
-    if (k == "k_initial") return L0 * (2.0 * 16.0 * D + LB + 4.0 + 4.0 + 1.0);  // + point copies
+    if (k == "k_initial")  // coordinates, slots + box, the four points, seed, error, flags
+        return L0 * (16.0 * D + SB + 16.0 * D + 4.0 + 4.0 + 1.0);
     if (k == "k_eval") {
         double n = (double)g.host.n_samples;
         return n * (4.0 * D + 12.0);
     }
     for (const auto& p : g.pstats_host) {
         if (k == "k_errors") {
-            if (&p == &g.pstats_host[0])  // first pass: every leaf's record, values, error
-                b += (double)p.stale * (LB + 4.0) + 12.0 * p.stale_vals;
+            if (&p == &g.pstats_host[0])  // first pass: every leaf's slots + box, values, error
+                b += (double)p.stale * (SB + 4.0) + 12.0 * p.stale_vals;
             else  // per split: list entry, split entry, two errors; the values
                 b += 0.5 * (double)p.stale * (4.0 + 32.0 + 8.0) + 12.0 * p.stale_vals;
         }
@@ -860,17 +862,16 @@ static double kernel_bytes(const Grid& g, const std::string& k) {
         else if (k == "k_scan_counts")
             b += (16.0 + 8.0) * g.chunk_blocks;
         else if (k == "k_place")  // compaction and placement
-            b += 6.0 * p.L +                                              // flags, states read; counts
-                                                                          // written
-                 (double)p.sel * (LB + 4.0 * D) +                         // record, new sample
-                 4.0 * D * p.selcnt +                                     // the leaf's points
-                 (double)(p.sel - p.split) * (16.0 + 4.0 * D) +           // slots + point of an insert
-                 (double)p.split * (2.0 * LB + 20.0 * D + 8.0 + 4.0 + 1.0 + 4.0 + 32.0);  // two
-                                                            // records and their points, errors, split entry
+            b += 6.0 * p.L +                                     // flags, states read; counts written
+                 (double)p.sel * (SB + 4.0 * D) +                // slots + box, the new sample
+                 4.0 * D * p.selcnt +                            // the leaf's points
+                 (double)(p.sel - p.split) * (4.0 + 4.0 * D) +   // slot entry + point of an insert
+                 (double)p.split * ((SB + 8.0 * D) + (SB + 12.0 * D) +  // both children's lines
+                                    4.0 + 1.0 + 4.0 + 32.0);     // list, flags, state, split entry
     }
     const double L = (double)g.host.n_leaves, N = (double)g.host.n_samples;
     const double K = (double)g.host.n_keys;
-    if (k == "k_recon_prep") return L * (LB + 32.0) + 12.0 * N;
+    if (k == "k_recon_prep") return L * (SB + 32.0) + 12.0 * N;
     if (k == "k_scan_keys") return (16.0 + 8.0) * g.chunk_blocks;
     if (k == "k_recon_emit") return L * 16.0 + K * 8.0;
     if (k == "k_rs_hist") return K * 8.0 * g.recon_sort_passes;

@@ -53,23 +53,27 @@ static const int kErrBlocks = 8192;     // max workgroups of the error kernels
 static const int kMaxPassStats = 4096;  // device ring of per-pass rows (a power of two); the
                                         // host drains it before it wraps
 
-// Float4 quads of one leaf record: the int4 of sample slots, then the box
-// (mn[0..D-1], mx[0..D-1]) padded to 16 bytes -- 48 bytes for D = 3, 4 and
-// 64 bytes for D = 5, 6, so that a leaf read at a random index touches one
-// cache-line segment.
-__host__ __device__ constexpr int leaf_quads(int D) { return 1 + (2 * D + 3) / 4; }
-
-// Floats per row of the per-leaf point copies: the 4 * D floats of the four
-// slots, padded to 64 bytes (D = 3, 4) or 128 bytes (D = 5, 6) so that a row
-// read or written at a random leaf stays inside one aligned block.
-__host__ __device__ constexpr int pts_stride(int D) { return 4 * D <= 16 ? 16 : 32; }
+// Leaf lines.  Each leaf owns one aligned 128-byte line (kLineFloats floats):
+// the int4 of sample slots, the box (mn[0..D-1], mx[0..D-1]) and the points of
+// its first line_pts(D) slots, D floats each, so that the placement reads and
+// writes a selected leaf in one line.  The points of the remaining slots (D = 5:
+// slot 3; D = 6: slots 2 and 3) sit in a per-leaf overflow row of
+// xpts_stride(D) floats (32 / 64 bytes), touched only by the last insert and
+// the split.  Float offsets in a line: slots 0..3, mn at 4, mx at 4 + D,
+// point q at 4 + 2 D + q D.
+static const int kLineFloats = 32;
+__host__ __device__ constexpr int line_pts(int D) { return D <= 4 ? 4 : (D == 5 ? 3 : 2); }
+__host__ __device__ constexpr int line_pt(int D, int q) { return 4 + 2 * D + q * D; }
+__host__ __device__ constexpr int xpts_stride(int D) { return D <= 4 ? 0 : (D == 5 ? 8 : 16); }
+// float4s of a line holding the slots and the box
+__host__ __device__ constexpr int line_box_quads(int D) { return (4 + 2 * D + 3) / 4; }
 
 // Plain-pointer view of the device state passed to kernels by value.
 struct DevGrid {
     float* coords;         // [cap_s * D]  AoS (the integrand contract)
     float* values;         // [cap_s * 3]  float3 AoS
-    float4* leaf;          // [cap_l * leaf_quads(D)]  slots (~D = empty) + box
-    float* pts;            // [cap_l * pts_stride(D)]  copy of each leaf's sample points by slot
+    float4* leaf;          // [cap_l * 8]  leaf lines: slots (~D = empty), box, first points
+    float* xpts;           // [cap_l * xpts_stride(D)]  the points of the other slots (D >= 5)
     float* err;            // [cap_l]      < 0 = stale
     unsigned* seed;        // [cap_l]      LCG state, 0 = unset
     unsigned char* flags;  // [cap_l]      bits 0-2 count, 3 selected, 4 split

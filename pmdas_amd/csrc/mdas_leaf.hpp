@@ -1,5 +1,6 @@
-// Access to the per-leaf records: one int4 of sample slots followed by the
-// box (mn[0..D-1], mx[0..D-1]), leaf_quads(D) float4 per leaf.
+// Access to the leaf lines (layout in mdas_internal.hpp): the int4 of sample
+// slots, the box (mn[0..D-1], mx[0..D-1]) and the first line_pts(D) points in
+// one aligned 128-byte line per leaf; the other points in the overflow rows.
 #pragma once
 #include "mdas_device.hpp"
 #include "mdas_internal.hpp"
@@ -8,78 +9,112 @@
 
 namespace mdas {
 
+// The whole line of leaf i (8 float4 loads of one 128-byte line).
+__device__ __forceinline__ void load_line(const DevGrid& a, int i, float* f) {
+    const float4* p = a.leaf + (size_t)i * (kLineFloats / 4);
+#pragma unroll
+    for (int k = 0; k < kLineFloats / 4; ++k) {
+        const float4 v = p[k];
+        f[4 * k] = v.x;
+        f[4 * k + 1] = v.y;
+        f[4 * k + 2] = v.z;
+        f[4 * k + 3] = v.w;
+    }
+}
+
+// Slots and box only: the line's first line_box_quads(D) float4.
 template <int D>
 __device__ __forceinline__ void load_leaf(const DevGrid& a, int i, int4& id, float* mn, float* mx) {
-    constexpr int Q = leaf_quads(D);
-    const float4* p = a.leaf + (size_t)i * Q;
-    const float4 q0 = p[0];
-    id = make_int4(__float_as_int(q0.x), __float_as_int(q0.y), __float_as_int(q0.z),
-                   __float_as_int(q0.w));
-    float b[4 * (Q - 1)];
+    constexpr int Q = line_box_quads(D);
+    const float4* p = a.leaf + (size_t)i * (kLineFloats / 4);
+    float b[4 * Q];
 #pragma unroll
-    for (int k = 1; k < Q; ++k) {
+    for (int k = 0; k < Q; ++k) {
         const float4 v = p[k];
-        b[4 * (k - 1) + 0] = v.x;
-        b[4 * (k - 1) + 1] = v.y;
-        b[4 * (k - 1) + 2] = v.z;
-        b[4 * (k - 1) + 3] = v.w;
+        b[4 * k] = v.x;
+        b[4 * k + 1] = v.y;
+        b[4 * k + 2] = v.z;
+        b[4 * k + 3] = v.w;
     }
+    id = make_int4(__float_as_int(b[0]), __float_as_int(b[1]), __float_as_int(b[2]), __float_as_int(b[3]));
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        mn[d] = b[d];
-        mx[d] = b[D + d];
+        mn[d] = b[4 + d];
+        mx[d] = b[4 + D + d];
     }
 }
 
 template <int D>
 __device__ __forceinline__ void load_box(const DevGrid& a, int i, float* mn, float* mx) {
-    constexpr int Q = leaf_quads(D);
-    const float4* p = a.leaf + (size_t)i * Q;
-    float b[4 * (Q - 1)];
-#pragma unroll
-    for (int k = 1; k < Q; ++k) {
-        const float4 v = p[k];
-        b[4 * (k - 1) + 0] = v.x;
-        b[4 * (k - 1) + 1] = v.y;
-        b[4 * (k - 1) + 2] = v.z;
-        b[4 * (k - 1) + 3] = v.w;
-    }
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        mn[d] = b[d];
-        mx[d] = b[D + d];
-    }
+    int4 id;
+    load_leaf<D>(a, i, id, mn, mx);
 }
 
-__device__ __forceinline__ int4 load_slots(const DevGrid& a, int i, int Q) {
-    const float4 q0 = a.leaf[(size_t)i * Q];
+__device__ __forceinline__ int4 load_slots(const DevGrid& a, int i) {
+    const float4 q0 = a.leaf[(size_t)i * (kLineFloats / 4)];
     return make_int4(__float_as_int(q0.x), __float_as_int(q0.y), __float_as_int(q0.z),
                      __float_as_int(q0.w));
 }
 
-__device__ __forceinline__ void store_slots(const DevGrid& a, int i, int Q, int4 id) {
-    a.leaf[(size_t)i * Q] = make_float4(__int_as_float(id.x), __int_as_float(id.y),
-                                        __int_as_float(id.z), __int_as_float(id.w));
+// Address of point q (slot order) of leaf i: in its line or its overflow row.
+template <int D>
+__device__ __forceinline__ float* point_addr(const DevGrid& a, int i, int q) {
+    return q < line_pts(D) ? reinterpret_cast<float*>(a.leaf) + (size_t)i * kLineFloats + line_pt(D, q)
+                           : a.xpts + (size_t)i * xpts_stride(D) + (q - line_pts(D)) * D;
 }
 
+// Writes a whole line: slots, box and the first points (pts[q] for q <
+// min(npts, line_pts(D))), zeros elsewhere, as 8 float4 stores.
 template <int D>
-__device__ __forceinline__ void store_leaf(const DevGrid& a, int i, int4 id, const float* mn,
-                                           const float* mx) {
-    constexpr int Q = leaf_quads(D);
-    float4* p = a.leaf + (size_t)i * Q;
-    float b[4 * (Q - 1)];
+__device__ __forceinline__ void store_line(const DevGrid& a, int i, int4 id, const float* mn,
+                                           const float* mx, const float (*pts)[D], int npts) {
+    float f[kLineFloats];
+    f[0] = __int_as_float(id.x);
+    f[1] = __int_as_float(id.y);
+    f[2] = __int_as_float(id.z);
+    f[3] = __int_as_float(id.w);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        b[d] = mn[d];
-        b[D + d] = mx[d];
+        f[4 + d] = mn[d];
+        f[4 + D + d] = mx[d];
     }
 #pragma unroll
-    for (int k = 2 * D; k < 4 * (Q - 1); ++k) b[k] = 0.0f;
-    p[0] = make_float4(__int_as_float(id.x), __int_as_float(id.y), __int_as_float(id.z),
-                       __int_as_float(id.w));
+    for (int k = 4 + 2 * D; k < kLineFloats; ++k) f[k] = 0.0f;
 #pragma unroll
-    for (int k = 1; k < Q; ++k)
-        p[k] = make_float4(b[4 * (k - 1)], b[4 * (k - 1) + 1], b[4 * (k - 1) + 2], b[4 * (k - 1) + 3]);
+    for (int q = 0; q < line_pts(D); ++q)
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            if (q < npts) f[line_pt(D, q) + d] = pts[q][d];
+    float4* p = a.leaf + (size_t)i * (kLineFloats / 4);
+#pragma unroll
+    for (int k = 0; k < kLineFloats / 4; ++k)
+        p[k] = make_float4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);
+}
+
+// Writes the line's first 4 + 4 D floats: slots, box and the first two
+// points (a left child keeps two), whole float4s; the rest of the line keeps
+// its old contents, which the slots mark empty.
+template <int D>
+__device__ __forceinline__ void store_line_head2(const DevGrid& a, int i, int4 id, const float* mn,
+                                                 const float* mx, const float* p0, const float* p1) {
+    constexpr int NF = 4 + 4 * D;  // 16, 20, 24, 28 floats: whole float4s
+    static_assert(NF % 4 == 0 && line_pts(D) >= 2, "two points in the line head");
+    float f[NF];
+    f[0] = __int_as_float(id.x);
+    f[1] = __int_as_float(id.y);
+    f[2] = __int_as_float(id.z);
+    f[3] = __int_as_float(id.w);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        f[4 + d] = mn[d];
+        f[4 + D + d] = mx[d];
+        f[line_pt(D, 0) + d] = p0[d];
+        f[line_pt(D, 1) + d] = p1[d];
+    }
+    float4* p = a.leaf + (size_t)i * (kLineFloats / 4);
+#pragma unroll
+    for (int k = 0; k < NF / 4; ++k)
+        p[k] = make_float4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);
 }
 
 // kdgrid.cu:237-266: relative-L1 error of a leaf's samples times its volume.
