@@ -186,15 +186,15 @@ def run_batch(steps, log=print):
 LEARNED = dict(train_seeds=200, train_W=160, train_H=90, spp=32, epochs=4, batch=65536, lr=1e-3,
                val_seeds=(1000, 1001, 1002, 1003), val_W=240, val_H=135,
                kappas=(1.0, 0.5, 0.25, 0.125, 0.0625),
-               eval_seeds=(2000, 2001, 2002, 2003), eval_W=480, eval_H=270,
-               eval_spp=(8, 12, 16, 24, 32, 64), gt_spp=4096)
+               eval_seeds=(2000, 2001, 2002, 2003), eval_W=1920, eval_H=1080,
+               eval_spp=(4, 5, 6, 8, 12, 16, 24, 32, 64), gt_spp=4096)
 
 
 def learned_comparison(log=print):
     """Learned proposal vs plain MDAS on the 5D moving-sphere scene: rows
     from plain runs of 200 seeded scenes, GPU training, the score exponent
     chosen on 4 validation scenes, then image MSE against an FP64 4096-spp
-    QMC ground truth on 4 held-out scenes; samples and time the learned form
+    QMC ground truth on 4 held-out 1920x1080 scenes; samples and time the learned form
     needs to reach plain's error at 32 spp, and the same for plain sampling
     with the learned form's effective exponent (the flattening alone)."""
     from pmdas_amd import learned as L
@@ -213,6 +213,14 @@ def learned_comparison(log=print):
     arms = ("plain", "learned", "plain_same_exponent")
     eq = {arm: {k: L.samples_to_equal(rows, 32, k, arm) for k in ("mse", "rel_mse")}
           for arm in arms[1:]}
+    # what the network adds over the flattened exponent alone: the learned
+    # form's samples and time to plain's 32-spp error, over those of plain
+    # sampling with the same effective exponent
+    lm, pm = eq["learned"]["mse"], eq["plain_same_exponent"]["mse"]
+    network_gain = None
+    if lm and pm:
+        network_gain = {"sample_ratio": round(lm["sample_ratio"] / pm["sample_ratio"], 3),
+                        "time_ratio": round(lm["time_ratio"] / pm["time_ratio"], 3)}
     per_spp = {}
     for spp in c["eval_spp"]:
         sel = [r for r in rows if r["spp"] == spp]
@@ -231,6 +239,11 @@ def learned_comparison(log=print):
             "eval": f"{len(c['eval_seeds'])} held-out seeds, {c['eval_W']}x{c['eval_H']} 5D, "
                     f"FP64 QMC {c['gt_spp']} spp ground truth",
             "per_spp": per_spp, "equal_error_at_32spp": eq,
+            "learned_vs_plain_same_exponent": network_gain,
+            "note": "sample_ratio / time_ratio: what each arm needs to reach plain MDAS's "
+                    "32-spp MSE (alpha 0.25), relative to plain; plain_same_exponent is plain "
+                    "MDAS with alpha * kappa, the learned form's flattening without the "
+                    "network; learned_vs_plain_same_exponent < 1 is what the network adds",
             "seconds": round(time.perf_counter() - t0, 1)}
 
 
@@ -530,6 +543,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-frames", action="store_true")
+    ap.add_argument("--no-device-record", action="store_true",
+                    help="skip rocminfo / rocm-smi (e.g. under a profiler whose preload "
+                         "would follow them into their child processes)")
     ap.add_argument("--cpu-shrink", type=int, default=0,
                     help="run the CPU baseline on a 2^-k per-axis version of the workload")
     a = ap.parse_args()
@@ -538,7 +554,8 @@ def main():
               "sharding by image or screen tile is a design, not built)", file=sys.stderr)
         sys.exit(2)
     log = lambda m: print(m, file=sys.stderr, flush=True)
-    dev_rec = device_record()  # child processes, before this one initialises the GPU
+    # child processes, before this one initialises the GPU
+    dev_rec = {} if a.no_device_record else device_record()
     log(f"[bench] device: {dev_rec}")
     copy_gbs = copy_probe()
     log(f"[bench] copy probe: {copy_gbs} GB/s")

@@ -300,10 +300,10 @@ __global__ __launch_bounds__(1024) void k_err_reduce_b(const DevGrid* grids, int
 // Writes the advanced LCG state and the flags, and per workgroup the
 // (selected, split) counts for the allocation scan, the samples held by the
 // selected leaves, and the max error of the leaves that do not split.
-__device__ __forceinline__ void select_body(const DevGrid& a, int blk, int nblk) {
+// Chunk blk of nblk (the compaction's partition of the leaves).
+__device__ __forceinline__ void select_chunk(const DevGrid& a, int blk, int nblk) {
     __shared__ int red[12];
     __shared__ float rmax[4];
-    if (a.cnt->done) return;
     const int L = a.cnt->n_leaves;
     const float gmax = __uint_as_float(a.cnt->gmax_bits);
     const float alpha = a.alpha;
@@ -364,11 +364,25 @@ __device__ __forceinline__ void select_body(const DevGrid& a, int blk, int nblk)
         a.part[blk] = make_int4(red[0] + red[1] + red[2] + red[3], red[4] + red[5] + red[6] + red[7],
                                 red[8] + red[9] + red[10] + red[11], (int)__float_as_uint(cm));
     }
+    __syncthreads();  // red / rmax are reused by the workgroup's next chunk
 }
-__global__ __launch_bounds__(256) void k_select(DevGrid a) { select_body(a, blockIdx.x, gridDim.x); }
-__global__ __launch_bounds__(256) void k_select_b(const DevGrid* grids) {
-    select_body(grids[blockIdx.y], blockIdx.x, gridDim.x);
+// Workgroup blk takes chunks [blk * cpw, (blk + 1) * cpw) of the compaction's
+// nblk * cpw: the placement balances better over many small chunks, the
+// selection's streaming sweep runs best with fewer, longer ones.
+__device__ __forceinline__ void select_body(const DevGrid& a, int blk, int nblk, int cpw) {
+    if (a.cnt->done) return;
+    for (int c = blk * cpw; c < (blk + 1) * cpw; ++c) select_chunk(a, c, nblk * cpw);
 }
+__global__ __launch_bounds__(256) void k_select(DevGrid a, int cpw) {
+    select_body(a, blockIdx.x, gridDim.x, cpw);
+}
+__global__ __launch_bounds__(256) void k_select_b(const DevGrid* grids, int cpw) {
+    select_body(grids[blockIdx.y], blockIdx.x, gridDim.x, cpw);
+}
+
+// Chunks per selection workgroup for `chunks` compaction chunks: as many as
+// keep the selection at 4096 workgroups (chunks is then a multiple of it).
+static int select_cpw(int chunks) { return chunks > 4096 && chunks % 4096 == 0 ? chunks / 4096 : 1; }
 
 // Exclusive scan of the per-workgroup counts (one workgroup of 1024 threads),
 // then the capacity check and the counter update (kdgrid.cu:753-759).
@@ -855,7 +869,8 @@ hipError_t launch_errors_stale(Grid& g, hipStream_t st) {
 }
 
 hipError_t launch_select(Grid& g, hipStream_t st) {
-    hipLaunchKernelGGL(k_select, dim3(g.chunk_blocks), dim3(256), 0, st, g.dev);
+    const int cpw = select_cpw(g.chunk_blocks);
+    hipLaunchKernelGGL(k_select, dim3(g.chunk_blocks / cpw), dim3(256), 0, st, g.dev, cpw);
     return hipGetLastError();
 }
 
@@ -924,7 +939,8 @@ hipError_t launch_pass_b(int D, const DevGrid* grids, int n, int blocks, hipStre
         MDAS_DISPATCH_D(D, hipLaunchKernelGGL(k_errors_stale_b<DD>, dim3(eb, n), dim3(256), 0, st, grids));
         hipLaunchKernelGGL(k_err_reduce_b, dim3(n), dim3(1024), 0, st, grids, eb);
     }
-    hipLaunchKernelGGL(k_select_b, dim3(blocks, n), dim3(256), 0, st, grids);
+    const int cpw = select_cpw(blocks);
+    hipLaunchKernelGGL(k_select_b, dim3(blocks / cpw, n), dim3(256), 0, st, grids, cpw);
     hipLaunchKernelGGL(k_scan_counts_b, dim3(n), dim3(1024), 0, st, grids, blocks);
     MDAS_DISPATCH_D(D, hipLaunchKernelGGL(k_compact_place_b<DD>, dim3(blocks, n), dim3(256), 0, st, grids));
     return hipGetLastError();

@@ -486,9 +486,11 @@ inline void make_scene(Params* P, int kind, uint32_t seed, int dim, float scale_
         }
         break;
     case SCENE_AO:
+        // the lens of the depth-of-field scenes (radius 0.05, focus distance
+        // 5); the camera steps back to z = -12 so that the [-5, 5]^3 box of
+        // spheres lies in front of it
         P->n_obj = 256;
         P->eye[2] = -12.0f;
-        P->focal_distance = 10.0f;
         P->plane_y = -5.5f;
         for (int k = 0; k < 256; ++k) {
             float* s = P->obj + k * OBJ_STRIDE;
