@@ -187,7 +187,7 @@ LEARNED = dict(train_seeds=200, train_W=160, train_H=90, spp=32, epochs=4, batch
                val_seeds=(1000, 1001, 1002, 1003), val_W=240, val_H=135,
                kappas=(1.0, 0.5, 0.25, 0.125, 0.0625),
                eval_seeds=(2000, 2001, 2002, 2003), eval_W=1920, eval_H=1080,
-               eval_spp=(4, 5, 6, 8, 12, 16, 24, 32, 64), gt_spp=4096)
+               eval_spp=(4, 8, 12, 16, 24, 32, 64), adaptive_spp=8, gt_spp=4096)
 
 
 def learned_comparison(log=print):
@@ -211,8 +211,15 @@ def learned_comparison(log=print):
     rows = L.compare(m, c["eval_seeds"], c["eval_W"], c["eval_H"], c["eval_spp"],
                      gt_spp=c["gt_spp"], kappa=kappa, log=log)
     arms = ("plain", "learned", "plain_same_exponent")
-    eq = {arm: {k: L.samples_to_equal(rows, 32, k, arm) for k in ("mse", "rel_mse")}
+    # 4 spp is the initial grid alone (the same image for every arm), so the
+    # comparisons use the adaptive points only
+    lo = c["adaptive_spp"]
+    eq = {arm: {k: L.samples_to_equal(rows, 32, k, arm, min_spp=lo) for k in ("mse", "rel_mse")}
           for arm in arms[1:]}
+    at_equal = {f"{a}_vs_{r}": {k: L.error_at_equal_samples(rows, a, r, k, min_spp=lo)
+                                for k in ("mse", "rel_mse")}
+                for a, r in (("learned", "plain"), ("plain_same_exponent", "plain"),
+                             ("learned", "plain_same_exponent"))}
     # what the network adds over the flattened exponent alone: the learned
     # form's samples and time to plain's 32-spp error, over those of plain
     # sampling with the same effective exponent
@@ -220,7 +227,8 @@ def learned_comparison(log=print):
     network_gain = None
     if lm and pm:
         network_gain = {"sample_ratio": round(lm["sample_ratio"] / pm["sample_ratio"], 3),
-                        "time_ratio": round(lm["time_ratio"] / pm["time_ratio"], 3)}
+                        "time_ratio": round(lm["time_ratio"] / pm["time_ratio"], 3),
+                        "bound": "upper" if "bound" in lm or "bound" in pm else None}
     per_spp = {}
     for spp in c["eval_spp"]:
         sel = [r for r in rows if r["spp"] == spp]
@@ -240,10 +248,14 @@ def learned_comparison(log=print):
                     f"FP64 QMC {c['gt_spp']} spp ground truth",
             "per_spp": per_spp, "equal_error_at_32spp": eq,
             "learned_vs_plain_same_exponent": network_gain,
-            "note": "sample_ratio / time_ratio: what each arm needs to reach plain MDAS's "
-                    "32-spp MSE (alpha 0.25), relative to plain; plain_same_exponent is plain "
-                    "MDAS with alpha * kappa, the learned form's flattening without the "
-                    "network; learned_vs_plain_same_exponent < 1 is what the network adds",
+            "error_at_equal_samples": at_equal,
+            "note": f"sample_ratio / time_ratio: what each arm needs to reach plain MDAS's "
+                    f"32-spp MSE (alpha 0.25), relative to plain, over the adaptive points "
+                    f"(spp >= {lo}; 'bound: upper' when the arm is below it at its first one); "
+                    f"error_at_equal_samples: an arm's error over another's at the other's "
+                    f"sample count; plain_same_exponent is plain MDAS with alpha * kappa, the "
+                    f"learned form's flattening without the network, so learned_vs_"
+                    f"plain_same_exponent < 1 is what the network adds",
             "seconds": round(time.perf_counter() - t0, 1)}
 
 

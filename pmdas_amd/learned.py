@@ -73,8 +73,9 @@ def heldout_loss(m, X, Y, n):
 
 def render(scene, W, H, spp, dim, mlp=None, b=1, image_label=False, alpha=0.25, kappa=1.0,
            kappa_y=None):
-    """MDAS image (H, W, 4), samples, sampling ms (device) for one scene.
-    kappa_y (optional) separates the network's exponent from kappa."""
+    """MDAS image (H, W, 4), samples, sampling + reconstruction ms (device;
+    the second of two identical runs) for one scene.  kappa_y (optional) separates the
+    network's exponent from kappa."""
     g, target = make_grid(W, H, spp, dim, b, alpha=alpha)
     if mlp is not None:
         g.set_learned(_lib.LEARN_SCORE | (_lib.LEARN_IMAGE_LABEL if image_label else 0), mlp)
@@ -82,6 +83,11 @@ def render(scene, W, H, spp, dim, mlp=None, b=1, image_label=False, alpha=0.25, 
             g.set_learned_exponents(0.0 if image_label else kappa, kappa_y)
         elif kappa != 1.0:
             g.set_learned_exponent(kappa)
+    # the first run and reconstruction size and touch the grid's buffers (the
+    # reconstruction's keys are allocated at its first call); the second
+    # ones are timed
+    g.Run(scene, target)
+    g.integrate_image(W, H)
     g.set_timing(True)
     g.Run(scene, target)
     img = g.integrate_image(W, H)
@@ -142,21 +148,35 @@ def compare(mlp, seeds, W, H, spps, dim=5, kind=SCENE_MBDOF, gt_spp=4096, image_
     return rows
 
 
-def samples_to_equal(rows, ref_spp, key="mse", arm="learned"):
+def _curve(rows, name, key, min_spp):
+    """Mean error, samples and time (sampling + integrand ms) of one arm over
+    seeds, per swept spp >= min_spp."""
+    spps = sorted({r["spp"] for r in rows if r["spp"] >= min_spp})
+    sel = [[r[name] for r in rows if r["spp"] == s] for s in spps]
+    e = np.array([np.mean([a[key] for a in x]) for x in sel])
+    n = np.array([np.mean([a["samples"] for a in x]) for x in sel])
+    t = np.array([np.mean([a["ms"] + a["integrand_ms"] for a in x]) for x in sel])
+    return spps, n, e, t
+
+
+def _loglog(xs, ys, x):
+    """log-log interpolation of y at x along (xs, ys) (xs increasing); None
+    outside the range."""
+    lx = np.log(xs)
+    if not (lx[0] <= math.log(x) <= lx[-1]):
+        return None
+    return float(np.exp(np.interp(math.log(x), lx, np.log(ys))))
+
+
+def samples_to_equal(rows, ref_spp, key="mse", arm="learned", min_spp=0):
     """Samples (and time) an arm needs to reach the plain method's mean error
     at ref_spp, relative to plain's there (log-log interpolation of the mean
-    error over seeds); None if the arm's curve never gets there."""
-    spps = sorted({r["spp"] for r in rows})
-
-    def curve(name):
-        e = [np.mean([r[name][key] for r in rows if r["spp"] == s]) for s in spps]
-        n = [np.mean([r[name]["samples"] for r in rows if r["spp"] == s]) for s in spps]
-        t = [np.mean([r[name]["ms"] + r[name]["integrand_ms"] for r in rows if r["spp"] == s])
-             for s in spps]
-        return np.array(n), np.array(e), np.array(t)
-
-    n_p, e_p, t_p = curve("plain")
-    n_l, e_l, t_l = curve(arm)
+    error over seeds).  Only swept points with spp >= min_spp enter (points
+    below it, e.g. the initial grid alone, are the same for every arm).  If
+    the arm is already below that error at its first point, that point is
+    returned as an upper bound; None if its curve never gets there."""
+    spps, n_p, e_p, t_p = _curve(rows, "plain", key, min_spp)
+    _, n_l, e_l, t_l = _curve(rows, arm, key, min_spp)
     k = spps.index(ref_spp)
     target = e_p[k]
     le, ln = np.log(e_l), np.log(n_l)
@@ -172,3 +192,22 @@ def samples_to_equal(rows, ref_spp, key="mse", arm="learned"):
         return dict(sample_ratio=float(n_l[0] / n_p[k]), time_ratio=float(t_l[0] / t_p[k]),
                     plain_err=float(target), plain_samples=float(n_p[k]), bound="upper")
     return None
+
+
+def error_at_equal_samples(rows, arm, ref_arm, key="mse", min_spp=0):
+    """The arm's mean error over the reference arm's at the reference arm's
+    sample count, per swept spp >= min_spp (the arm's error interpolated
+    log-log in its samples; points outside the arm's sampled range are
+    left out), and their geometric mean.  Sampling passes overshoot their
+    target by different amounts per arm, so equal targets are not equal
+    samples; this compares at equal samples."""
+    spps, n_r, e_r, _ = _curve(rows, ref_arm, key, min_spp)
+    _, n_a, e_a, _ = _curve(rows, arm, key, min_spp)
+    order = np.argsort(n_a)
+    ratios = {}
+    for s, n, e in zip(spps, n_r, e_r):
+        v = _loglog(n_a[order], e_a[order], n)
+        if v is not None:
+            ratios[str(s)] = v / float(e)
+    gm = float(np.exp(np.mean(np.log(list(ratios.values()))))) if ratios else None
+    return dict(ratios=ratios, geomean=gm)

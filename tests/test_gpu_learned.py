@@ -40,6 +40,40 @@ def _params(seed, scale=1.0):
     return p
 
 
+def _flip_bound(p, X, margin=4e-3):
+    """The oracle's outputs for rows X and, per row, a bound on how far a
+    correct GPU evaluation can differ from them.  Both round the layer inputs
+    to bf16; the GPU accumulates in float32 (MFMA) and takes the GELU's
+    quotient on the hardware reciprocal, so a pre-rounding activation can
+    land on the other side of a bf16 rounding tie when it lies within a
+    small fraction of an ulp of one (`margin`, in ulps; the float32
+    differences are ~1e-3 of that).  Each such flip moves that activation by
+    one bf16 ulp: a second-layer flip moves the output by |w3_k| ulp; a
+    first-layer flip moves every z2 by |W2_kj| ulp to first order (GELU'
+    and w3 carry it on), and can flip any second-layer value whose tie lies
+    within that move.  Rows with no value near a tie must agree to float32
+    accumulation error."""
+    y, c = mo.forward(p, X, keep=True)
+
+    def parts(a):
+        a32 = np.asarray(a, np.float32)
+        u = a32.view(np.uint32)
+        frac = (u & 0xFFFF).astype(np.float64) / 65536.0
+        ulp = np.ldexp(1.0, ((u >> 23) & 0xFF).astype(np.int64) - 127 - 7)
+        return np.abs(frac - 0.5), ulp
+    t1, u1 = parts(mo.gelu(c["z1"]))
+    t2, u2 = parts(mo.gelu(c["z2"]))
+    d1 = np.where(t1 < margin, u1, 0.0)  # first-layer flips, (n, 64)
+    d2 = np.where(t2 < margin, u2, 0.0)
+    w3, W2 = np.abs(c["w3"]), np.abs(c["W2"])
+    dz2 = d1 @ W2.T
+    move2 = np.abs(mo.gelu_grad(c["z2"])) * dz2  # how far each a2 moves
+    sec = np.where(t2 * u2 <= 1.5 * move2, u2, 0.0)  # flips that move can cause
+    bound = (d2 + sec + 1.5 * move2) @ w3
+    floor = 1e-5 * (1.0 + np.abs(c["a2"]) @ w3)
+    return y, bound + floor, (d1 > 0).any(1) | (d2 > 0).any(1)
+
+
 @pytest.mark.parametrize("n", [1, 16, 37, 1000])
 def test_mlp_forward_matches_oracle(mdas, n):
     _say(f"mlp forward n={n}")
@@ -49,9 +83,12 @@ def test_mlp_forward_matches_oracle(mdas, n):
     m.set_params(p)
     assert np.array_equal(m.params(), p)
     y = m.forward(X)
-    want = mo.forward(p, X)
+    want, bound, near = _flip_bound(p, X)
     d = np.abs(y - want)
-    assert np.median(d) < 1e-4 and d.max() < 5e-2, (np.median(d), d.max())
+    _say(f"  rows near a bf16 tie {near.mean():.3f}, max diff {d.max():.2e}, "
+         f"max diff / bound {np.max(d / bound):.3f}")
+    assert (d <= bound).all(), (np.argmax(d / bound), float(np.max(d / bound)))
+    assert np.median(d) < 1e-4
     m.close()
 
 
@@ -231,10 +268,17 @@ def test_learned_scores_match_oracle(mdas):
     st = g.download_state()
     f, pe = g.leaf_features()
     live = st["err"] >= 0
-    y = np.clip(mo.forward(p, f[live]), -10, 0)
+    y, bound, near = _flip_bound(p, f[live])
+    y = np.clip(y, -10, 0)
     want = pe[live].astype(np.float64) * np.exp2(y)
     rel = np.abs(st["err"][live] - want) / want
-    assert np.median(rel) < 1e-4 and rel.max() < 5e-2, (np.median(rel), rel.max())
+    # score = perr * 2^y: a change dy of the output moves it by ln2 * dy
+    # relative, plus the float32 rounding of the score itself
+    lim = np.log(2.0) * bound * 1.001 + 2e-6
+    _say(f"  leaves near a bf16 tie {near.mean():.3f}, max rel {rel.max():.2e}, "
+         f"max rel / bound {np.max(rel / lim):.3f}")
+    assert (rel <= lim).all(), (int(np.argmax(rel / lim)), float(np.max(rel / lim)))
+    assert np.median(rel) < 1e-4
     m.close()
 
 
