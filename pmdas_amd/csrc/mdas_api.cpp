@@ -750,6 +750,9 @@ int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream)
     HIP_TRY(mdas::launch_eval(g, s->dev, 0, 4 * L0, false, st));
     mdas::kend(g, st);
     long long last_n = -1;  // samples after the last block of passes
+    // every timed launch in order (name, ms); folded after each block, so the
+    // event pool holds one block's events however many passes the run takes
+    std::vector<std::pair<std::string, double>> seq;
     struct LoopMode {  // the pass kernels of this loop skip the stale markers
         mdas::DevGrid& d;
         explicit LoopMode(mdas::DevGrid& x) : d(x) { d.in_loop = 1; }
@@ -777,6 +780,7 @@ int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream)
         int rc = sync_counters(g, st);
         if (rc == MDAS_OK) rc = mdas::drain_pass_stats(g, false);
         if (rc) return rc;
+        if (g.timing) collect_kernel_times(g, logging ? &seq : nullptr);
         if (g.host.done || g.host.status) break;
         // every pass selects the leaf of maximum error (its ratio is 1), so
         // each block adds samples and the loop ends at the limit; a block that
@@ -791,7 +795,6 @@ int mdas_run(mdas_handle h, mdas_scene_handle s, long long target, void* stream)
     HIP_TRY(mdas::launch_mark_last_children(g, st));
     mdas::kend(g, st);
     g.passes = g.host.passes;
-    std::vector<std::pair<std::string, double>> seq;
     collect_kernel_times(g, logging ? &seq : nullptr);
     {
         int rc = mdas::drain_pass_stats(g, true);

@@ -126,7 +126,7 @@ def test_zero_scores_stop_with_state_error(mdas):
 
 def test_pass_rows_past_the_ring(mdas, tmp_path):
     """7573 passes (more than the 4096-row device ring): every pass keeps its
-    row, and the log has a nonzero SAMPLES line per pass."""
+    row, and the log has a nonzero SAMPLES line and a time per pass."""
     target = 60000
     args = (3, int(target * 1.6) + 68, 4, 0, 0, 1.0, 1.0, 1.0)
     sc = mdas.Scene(1, 3, 3, 1.0, 1.0)
@@ -143,5 +143,14 @@ def test_pass_rows_past_the_ring(mdas, tmp_path):
     samples = [int(float(lines[i + 1])) for i in range(len(lines) - 1) if lines[i] == "SAMPLES"]
     assert len(samples) == passes and min(samples) > 0
     assert samples == rows[:, 2].tolist()
+    # the timed launches are folded block by block: every pass keeps its
+    # time, and the logged error-pass times add up to the kernel statistics'
+    # total (which also counts the no-op passes after the limit in the last
+    # block of passes)
+    err_t = [float(lines[i + 1]) for i in range(len(lines) - 1) if lines[i] == "COMPUTE ERRORS TIME"]
+    assert len(err_t) == passes and min(err_t) > 0.0
+    launches, total_ms, _ = g.kernel_stats()["k_errors"]
+    assert launches >= passes
+    assert 0.9 * total_ms <= sum(err_t) <= total_ms * (1 + 1e-4) + 1e-3
     g.close()
     sc.close()
