@@ -469,6 +469,13 @@ def roofline(res, traffic_file, copy_gbs=None):
                avg_launch_ms=k["ms"] / max(k["launches"], 1), launches=k["launches"],
                copy_probe_GBps=copy_gbs,
                frac_of_copy=round(achieved / copy_gbs, 4) if copy_gbs else None)
+    if traffic:
+        # the counted HBM traffic (whole 128-byte lines for the scattered
+        # accesses) over the same launch time: how close the kernel runs to
+        # the bandwidth a plain copy reaches
+        tr = traffic / (out["avg_launch_ms"] * 1e-3) / 1e9
+        out["traffic_GBps"] = round(tr, 1)
+        out["traffic_frac_of_copy"] = round(tr / copy_gbs, 4) if copy_gbs else None
     # the whole run under the reference kernels' per-phase bytes: what those
     # kernels would move, over this build's MDAS phase time
     pb = res.get("phase_model_bytes")

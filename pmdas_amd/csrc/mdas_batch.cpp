@@ -54,7 +54,11 @@ struct mdas_batch {
     std::vector<mdas_handle> grids;
     std::vector<mdas_scene_handle> scenes;
     std::vector<Group> groups;
+    // the stream a null-stream caller's graphs are captured and launched on
+    // (the null stream cannot be captured), joined to the null stream on
+    // both sides by the two events
     hipStream_t own_stream = nullptr;
+    hipEvent_t own_in = nullptr, own_out = nullptr;
     int passes_per_block = 4;
     double last_ms = 0.0;
     int last_blocks = 0;
@@ -67,6 +71,31 @@ struct mdas_batch {
 };
 
 namespace {
+// The stream a call runs on: the caller's, or for a null-stream caller that
+// wants graphs the batch's own stream, made to wait for the work already on
+// the null stream (a non-blocking stream does not by itself).
+hipError_t enter_stream(mdas_batch* b, void* stream, bool use_graph, hipStream_t* st) {
+    *st = (hipStream_t)stream;
+    if (!use_graph || stream) return hipSuccess;
+    hipError_t e = hipSuccess;
+    if (!b->own_stream) e = hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking);
+    if (e == hipSuccess && !b->own_in) e = hipEventCreateWithFlags(&b->own_in, hipEventDisableTiming);
+    if (e == hipSuccess && !b->own_out) e = hipEventCreateWithFlags(&b->own_out, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(b->own_in, nullptr);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->own_stream, b->own_in, 0);
+    if (e == hipSuccess) *st = b->own_stream;
+    return e;
+}
+
+// After a call that ran on the batch's own stream: later work on the null
+// stream waits for it, as it would had the call run there.
+hipError_t leave_stream(mdas_batch* b, hipStream_t st) {
+    if (!b->own_stream || st != b->own_stream) return hipSuccess;
+    hipError_t e = hipEventRecord(b->own_out, st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(nullptr, b->own_out, 0);
+    return e;
+}
+
 void drop_recon_graph(mdas_batch* b) {
     if (b->recon_exec) (void)hipGraphExecDestroy(b->recon_exec);
     if (b->recon_graph) (void)hipGraphDestroy(b->recon_graph);
@@ -143,6 +172,8 @@ int mdas_batch_destroy(mdas_batch_handle b) {
         if (g.d_learn) (void)hipFree(g.d_learn);
     }
     drop_recon_graph(b);
+    if (b->own_in) (void)hipEventDestroy(b->own_in);
+    if (b->own_out) (void)hipEventDestroy(b->own_out);
     if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
     delete b;
     return MDAS_OK;
@@ -154,14 +185,11 @@ int mdas_batch_destroy(mdas_batch_handle b) {
 // caller passes the null stream, which cannot be captured).
 int mdas_batch_run(mdas_batch_handle b, const long long* targets, void* stream, int use_graph) {
     if (!b || !targets) return mdas::set_error(MDAS_ERR_INVALID, "null argument");
-    hipStream_t st = (hipStream_t)stream;
-    if (use_graph && !st) {
-        if (!b->own_stream) B_TRY(hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking));
-        st = b->own_stream;
-    }
     const int n = (int)b->grids.size();
     for (int i = 0; i < n; ++i)
         if (targets[i] < 0) return mdas::set_error(MDAS_ERR_INVALID, "negative target");
+    hipStream_t st = nullptr;
+    B_TRY(enter_stream(b, stream, use_graph != 0, &st));
     const auto t0 = std::chrono::steady_clock::now();
     const int B = std::max(1, b->passes_per_block);
     // initial grids and their integrand, one launch per kernel per group
@@ -258,6 +286,7 @@ int mdas_batch_run(mdas_batch_handle b, const long long* targets, void* stream, 
         rc = mdas::drain_pass_stats(gg, true);
         if (rc) return rc;
     }
+    B_TRY(leave_stream(b, st));
     b->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     b->last_blocks = blocks_run;
     return MDAS_OK;
@@ -280,11 +309,8 @@ int mdas_batch_integrate(mdas_batch_handle b, float* const* pixels, const int* w
             return mdas::set_error(MDAS_ERR_INVALID, "image size out of range at " + std::to_string(i));
         if (!g.initialized) return mdas::set_error(MDAS_ERR_STATE, "image " + std::to_string(i) + " has no samples");
     }
-    hipStream_t st = (hipStream_t)stream;
-    if (use_graph && !st) {
-        if (!b->own_stream) B_TRY(hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking));
-        st = b->own_stream;
-    }
+    hipStream_t st = nullptr;
+    B_TRY(enter_stream(b, stream, use_graph != 0, &st));
     bool bounded = true;
     for (int i = 0; i < n; ++i) {
         mdas::Grid& g = b->grids[i]->g;
@@ -296,6 +322,7 @@ int mdas_batch_integrate(mdas_batch_handle b, float* const* pixels, const int* w
             const int rc = mdas_integrate(b->grids[i], pixels[i], widths[i], heights[i], st);
             if (rc) return rc;
         }
+        B_TRY(leave_stream(b, st));
         return MDAS_OK;
     }
     std::vector<unsigned long long> key;
@@ -335,6 +362,7 @@ int mdas_batch_integrate(mdas_batch_handle b, float* const* pixels, const int* w
         b->recon_key = key;
     }
     B_TRY(hipGraphLaunch(b->recon_exec, st));
+    B_TRY(leave_stream(b, st));
     return MDAS_OK;
 }
 

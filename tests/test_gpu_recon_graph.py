@@ -8,7 +8,13 @@ synchronisation outside its logging).
 - mdas_batch_integrate reconstructs every image of a batch as one captured
   graph.  The result equals each grid's own mdas_integrate, also on a replay
   after the batch has sampled again, and without the graph.
+- A null-stream caller's batch graphs run on the batch's own stream, joined
+  to the null stream on both sides: a plain hipMemcpy right after the call
+  reads the finished image, and null-stream work issued before the call is
+  done before the graph writes.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -95,6 +101,38 @@ def test_batch_integrate_graph_equals_each_image(mdas):
             for i, (buf, (W, H)) in enumerate(zip(bufs, sizes)):
                 got = buf.download(np.float32, W * H * 4).reshape(H, W, 4)
                 assert np.array_equal(got.view(np.uint32), want[i].view(np.uint32)), (rnd, use_graph, i)
+    b.close()
+    for x in bufs + grids + scenes:
+        x.close() if hasattr(x, "close") else x.free()
+
+
+def test_batch_graph_keeps_null_stream_order(mdas):
+    from pmdas_amd import hip
+    specs = [(5, 192, 108, 16, 3, 7), (4, 160, 96, 8, 2, 8)]
+    grids, scenes, targets, sizes = [], [], [], []
+    for dim, W, H, spp, kind, seed in specs:
+        g, sc, t = _grid(mdas, dim, W, H, spp, kind, seed)
+        grids.append(g)
+        scenes.append(sc)
+        targets.append(t)
+        sizes.append((W, H))
+    b = mdas.Batch(grids, scenes)
+    b.run(targets)  # null stream, graphs on the batch's stream
+    want = [g.integrate_image(W, H) for g, (W, H) in zip(grids, sizes)]
+    bufs = [hip.DeviceBuffer(W * H * 16) for W, H in sizes]
+    Ws, Hs = [s[0] for s in sizes], [s[1] for s in sizes]
+    for rep in range(3):
+        # NaN bytes written on the null stream just before the call: the graph
+        # must run after them
+        for buf, (W, H) in zip(bufs, sizes):
+            assert hip.lib().hipMemsetAsync(ctypes.c_void_p(buf.ptr), ctypes.c_int(0xFF),
+                                            ctypes.c_size_t(W * H * 16), None) == 0
+        b.integrate(bufs, Ws, Hs, use_graph=True)
+        # no synchronisation: the download (hipMemcpy on the null stream)
+        # must wait for the graph
+        for i, (buf, (W, H)) in enumerate(zip(bufs, sizes)):
+            got = buf.download(np.float32, W * H * 4).reshape(H, W, 4)
+            assert np.array_equal(got.view(np.uint32), want[i].view(np.uint32)), (rep, i)
     b.close()
     for x in bufs + grids + scenes:
         x.close() if hasattr(x, "close") else x.free()
