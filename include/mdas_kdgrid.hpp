@@ -91,7 +91,15 @@ public:
         c.scale_y = scaleY;
         c.log_path = nullptr;
         check(mdas_create(&c, &h_));
-        if (log) check(mdas_set_log_sink(h_, &ofstream_log_sink, log));
+        if (log) {
+            const int rc = mdas_set_log_sink(h_, &ofstream_log_sink, log);
+            if (rc < 0) {  // no destructor runs for a constructor that throws
+                const std::string m = mdas_last_error();
+                mdas_destroy(h_);
+                h_ = nullptr;
+                throw Error(rc, m);
+            }
+        }
     }
     ~KDGrid() { mdas_destroy(h_); }
     KDGrid(const KDGrid&) = delete;
