@@ -60,6 +60,10 @@ int main(int argc, char** argv) {
                            MDAS_SCENE_SPHERES_MBDOF, MDAS_SCENE_AO};
     int kind = env.scene_kind >= 0 ? env.scene_kind : by_dim[dim >= 3 && dim <= 6 ? dim : 5];
     long long max_samples = 30000000;  // optixMotionBlur.cpp:81
+    if ((argc - 2) % 2 != 0) {
+        std::fprintf(stderr, "mdas_render: option %s has no value\n", argv[argc - 1]);
+        return 2;
+    }
     for (int i = 2; i + 1 < argc; i += 2) {
         const std::string k = argv[i];
         if (k == "--out") out = argv[i + 1];

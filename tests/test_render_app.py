@@ -30,6 +30,18 @@ def test_usage_without_arguments():
     assert r.returncode == 2 and "usage" in r.stderr
 
 
+def test_bad_options_are_refused():
+    """An option without a value or an unknown option ends the run with
+    status 2 before anything touches the GPU."""
+    if not os.path.exists(EXE):
+        pytest.skip("mdas_render not built")
+    env = os.path.join(GOLDEN, "pool.env")
+    r = subprocess.run([EXE, env, "--seed", "3", "--out"], capture_output=True, text=True)
+    assert r.returncode == 2 and "--out has no value" in r.stderr
+    r = subprocess.run([EXE, env, "--bogus", "1"], capture_output=True, text=True)
+    assert r.returncode == 2 and "unknown option --bogus" in r.stderr
+
+
 def _read_pfm(path):
     raw = open(path, "rb").read()
     head = raw.split(b"\n", 3)
