@@ -297,9 +297,11 @@ __global__ __launch_bounds__(1024) void k_err_reduce_b(const DevGrid* grids, int
 // --------------------------------------------------------------- selection --
 // kdgrid.cu:316-326.  Per leaf: the seed (tea4(i,0) when unset), one draw u,
 // selected iff pow(err / g_error, alpha) >= u.  A selected full leaf splits.
-// Writes the advanced LCG state and the flags, and per workgroup the
-// (selected, split) counts for the allocation scan, the samples held by the
-// selected leaves, and the max error of the leaves that do not split.
+// Writes the flags, and per workgroup the (selected, split) counts for the
+// allocation scan, the samples held by the selected leaves, and the max error
+// of the leaves that do not split.  The LCG states are not written here: the
+// compaction stores them once the scan has accepted the pass, so that a pass
+// refused for capacity leaves every state as it was.
 // Chunk blk of nblk (the compaction's partition of the leaves).
 __device__ __forceinline__ void select_chunk(const DevGrid& a, int blk, int nblk) {
     __shared__ int red[12];
@@ -337,11 +339,6 @@ __device__ __forceinline__ void select_chunk(const DevGrid& a, int blk, int nblk
             const unsigned cnt = fv[r] & 7u;
             const bool split = sel && cnt == 4u;
             a.flags[i] = (unsigned char)(cnt | (sel ? 8u : 0u) | (split ? 16u : 0u));
-            // a selected leaf's state after the placement's candidatesNum * D
-            // draws, by the LCG's affine jump-ahead; the compaction recovers
-            // the state the placement starts from with the inverse map
-            if (sel) s = a.jmul * s + a.jadd;
-            a.seed[i] = s;
             ns += sel ? 1 : 0;
             nl += split ? 1 : 0;
             nc += sel ? (int)cnt : 0;
@@ -683,8 +680,8 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
 // L_old + q.  The entries (leaf | count << 30, right child or -1; leaf
 // indices are below 2^30, a split leaf's count is 4) and the LCG states
 // they start from queue up in LDS and are placed a round of 256 at a time,
-// one per thread.  Every leaf's count is rewritten in the same sweep, as it
-// stands after the placement, in full coalesced lines.
+// one per thread.  Every leaf's count and LCG state are rewritten in the
+// same sweep, as they stand after the placement, in full coalesced lines.
 template <int D>
 __device__ __forceinline__ void compact_place_body(const DevGrid& a, int blk, int nblk) {
     constexpr int U = 4;                   // tiles whose flags and states are loaded together
@@ -726,13 +723,21 @@ __device__ __forceinline__ void compact_place_body(const DevGrid& a, int blk, in
             const unsigned cnt = f & 7u;
             int rs, rl, ts, tl;
             block_rank2(sel, split, lds, rs, rl, ts, tl);
+            // the state after the selection's draw (kdgrid.cu:323-326): the
+            // placement starts from it, and a selected leaf's stored state is
+            // the one after the candidatesNum * D candidate draws (the LCG's
+            // affine jump-ahead)
+            unsigned s1 = sv[r];
+            if (s1 == 0u) s1 = tea4((unsigned)i, 0u);
+            s1 = 1664525u * s1 + 1013904223u;
             if (sel) {
                 qe[qn + rs] = make_int2(i | (int)((cnt & 3u) << 30), split ? L + base.y + rl : -1);
-                // the selection stored the state after the candidate draws;
-                // the placement starts from the state before them
-                qs[qn + rs] = a.jinv * (sv[r] - a.jadd);
+                qs[qn + rs] = s1;
             }
-            if (i < end) a.flags[i] = (unsigned char)(split ? 2u : (sel ? cnt + 1u : cnt));
+            if (i < end) {
+                a.flags[i] = (unsigned char)(split ? 2u : (sel ? cnt + 1u : cnt));
+                a.seed[i] = sel ? a.jmul * s1 + a.jadd : s1;
+            }
             qn += ts;
             base.y += tl;
         }

@@ -42,18 +42,14 @@ hipError_t dalloc(T** p, size_t n) {
 }
 
 // The LCG x -> 1664525 x + 1013904223 (mod 2^32) applied k times is
-// x -> A x + C; A is odd, so the map is invertible: x = A^-1 (y - C), with
-// A^-1 by Newton's iteration mod 2^32.
-void lcg_jump(unsigned k, unsigned& A, unsigned& C, unsigned& Ainv) {
+// x -> A x + C.
+void lcg_jump(unsigned k, unsigned& A, unsigned& C) {
     A = 1u;
     C = 0u;
     for (unsigned j = 0; j < k; ++j) {
         A = 1664525u * A;
         C = 1664525u * C + 1013904223u;
     }
-    unsigned x = A;  // correct to 3 bits for odd A
-    for (int it = 0; it < 5; ++it) x *= 2u - A * x;
-    Ainv = x;
 }
 
 void free_grid(Grid& g) {
@@ -241,7 +237,7 @@ int mdas_create(const mdas_config* cfg, mdas_handle* out) {
     d.cap_s = g.cap_s;
     d.cap_l = g.cap_l;
     d.cand = g.cand;
-    lcg_jump((unsigned)(g.cand * g.D), d.jmul, d.jadd, d.jinv);
+    lcg_jump((unsigned)(g.cand * g.D), d.jmul, d.jadd);
     d.b = g.b;
     d.e = g.e;
     d.alpha = g.alpha;
@@ -860,12 +856,12 @@ static double kernel_bytes(const Grid& g, const std::string& k) {
             else  // per split: list entry, split entry, two errors; the values
                 b += 0.5 * (double)p.stale * (4.0 + 32.0 + 8.0) + 12.0 * p.stale_vals;
         }
-        else if (k == "k_select")  // error, state, flags read; state, flags written
-            b += 14.0 * p.L;
+        else if (k == "k_select")  // error, state, flags read; flags written
+            b += 10.0 * p.L;
         else if (k == "k_scan_counts")
             b += (16.0 + 8.0) * g.chunk_blocks;
         else if (k == "k_place")  // compaction and placement
-            b += 6.0 * p.L +                                     // flags, states read; counts written
+            b += 10.0 * p.L +                                    // flags, states read and written
                  (double)p.sel * (SB + 4.0 * D) +                // slots + box, the new sample
                  4.0 * D * p.selcnt +                            // the leaf's points
                  (double)(p.sel - p.split) * (4.0 + 4.0 * D) +   // slot entry + point of an insert

@@ -92,7 +92,7 @@ struct DevGrid {
     int D;
     int cap_s, cap_l;
     int cand;
-    unsigned jmul, jadd, jinv;  // LCG jump over candidatesNum * D draws and its inverse
+    unsigned jmul, jadd;  // LCG jump over candidatesNum * D draws
     int in_loop;                // 1 inside the device-side loop: the placement leaves the
                                 //   children's stale markers to the next error pass
     int b, e;

@@ -224,14 +224,31 @@ def test_capacity_error_is_reported(mdas):
     sc = mdas.Scene(0, 0, 3, 8.0, 8.0)
     g.InitialSampling()
     g.EvalIntegrand(sc, 0, 256)
-    before = g.download_state()
     g.ComputeErrors()
+    before = g.download_state()
     with pytest.raises(MdasError) as ei:
         g.AdaptiveSampling()
     assert ei.value.code == -2
+    # the refused pass changed nothing, LCG states included, and the state is
+    # the oracle's after its own refused pass
     after = g.download_state()
-    for k in ("coords", "idx", "box"):
-        assert np.array_equal(before[k], after[k])
+    assert set(after) == {"coords", "values", "idx", "box", "err", "seed"}
+    for k in after:
+        assert np.array_equal(before[k], after[k]), k
+    o = pyoracle.OracleGrid(3, 300, 4, 0, 3, 0.0, 8.0, 8.0)
+    o.initial_sampling()
+    o.eval_scene(sc.params_bytes(), 0, 256)
+    o.compute_errors()
+    assert o.adaptive_sampling() != 0
+    for k, v in o.state().items():
+        assert np.array_equal(after[k], v), k
+    # a second attempt draws the same numbers and is refused the same way
+    with pytest.raises(MdasError):
+        g.AdaptiveSampling()
+    again = g.download_state()
+    for k in after:
+        assert np.array_equal(after[k], again[k]), k
+    o.close()
     # the device-side loop never starts a pass that could overflow (N + L =
     # 320 > 300): it ends cleanly with the initial grid
     assert g.Run(sc, 10**6) == 0 and g.GetNumberOfSamples() == 256
