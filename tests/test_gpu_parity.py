@@ -254,6 +254,43 @@ def test_capacity_error_is_reported(mdas):
     assert g.Run(sc, 10**6) == 0 and g.GetNumberOfSamples() == 256
 
 
+def test_refused_pass_mid_run_then_retry(mdas):
+    """A pass refused for capacity after a successful one: the state is the
+    oracle's after its own refused pass; ComputeErrors again (the retry a
+    SamplingPass makes) gives the oracle's errors and maximum, and the pass
+    is refused again with nothing changed."""
+    from pmdas_amd import MdasError
+    args = (3, 400, 4, 0, 3, 0.0, 8.0, 8.0)  # 64 leaves; pass 1 -> 320 samples, pass 2 -> 448
+    g = mdas.KDGrid(*args)
+    o = pyoracle.OracleGrid(*args)
+    sc = mdas.Scene(0, 5, 3, 8.0, 8.0)
+    params = sc.params_bytes()
+    g.InitialSampling()
+    o.initial_sampling()
+    g.EvalIntegrand(sc, 0, 256)
+    o.eval_scene(params, 0, 256)
+    g.SamplingPass()
+    assert o.sampling_pass() == 0
+    n = g.GetNumberOfSamples()
+    assert n == o.n_samples == 320
+    g.EvalIntegrand(sc, 256, n - 256)
+    o.eval_scene(params, 256, n - 256)
+    for attempt in range(2):
+        g.ComputeErrors()
+        o.compute_errors()
+        assert g.GetMaxError() == o.max_error, attempt
+        with pytest.raises(MdasError) as ei:
+            g.AdaptiveSampling()
+        assert ei.value.code == -2
+        assert o.adaptive_sampling() != 0
+        gs, os_ = g.download_state(), o.state()
+        for k in os_:
+            assert np.array_equal(gs[k], os_[k]), (attempt, k)
+    g.close()
+    o.close()
+    sc.close()
+
+
 def test_upload_state_then_pass(mdas):
     """A pass on an uploaded mid-run state matches the oracle on the same state."""
     g, o, sc, target = _make(mdas, 3, 5, 1, 2, 48, 32, 8, 0.25, 4)
