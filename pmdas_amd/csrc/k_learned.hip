@@ -344,7 +344,7 @@ __device__ __forceinline__ void err_reduce_learned_body(const DevGrid& a, const 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned mm = a.cnt->gmax_bits;
+        unsigned mm = a.cnt->carry_bits;  // the loop's pass: from the scan's carried max
         int s1 = 0, s2 = 0;
         for (int k = 0; k < 16; ++k) {
             mm = max(mm, sm[k]);

@@ -247,8 +247,12 @@ __global__ __launch_bounds__(256) void k_errors_stale_b(const DevGrid* grids) {
     errors_stale_body<D>(grids[blockIdx.y], blockIdx.x, gridDim.x);
 }
 
-// Folds the error partials into gmax and the per-pass statistics.
-__device__ __forceinline__ void err_reduce_body(const DevGrid& a, int G) {
+// Folds the error partials into gmax and the per-pass statistics.  A full
+// sweep (ComputeErrors, after its reset) starts from gmax = 0; the loop's
+// stale-list pass, which re-estimates only the last pass's children, starts
+// from the max the allocation scan carried over (the leaves that did not
+// split).
+__device__ __forceinline__ void err_reduce_body(const DevGrid& a, int G, int from_carry) {
     __shared__ unsigned sm[16];
     __shared__ int ss[16], sv[16];
     if (a.cnt->done) return;
@@ -277,7 +281,7 @@ __device__ __forceinline__ void err_reduce_body(const DevGrid& a, int G) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned mm = a.cnt->gmax_bits;
+        unsigned mm = from_carry ? a.cnt->carry_bits : a.cnt->gmax_bits;
         int s1 = 0, s2 = 0;
         for (int k = 0; k < 16; ++k) {
             mm = max(mm, sm[k]);
@@ -289,9 +293,11 @@ __device__ __forceinline__ void err_reduce_body(const DevGrid& a, int G) {
         a.cnt->stale_vals += s2;
     }
 }
-__global__ __launch_bounds__(1024) void k_err_reduce(DevGrid a, int G) { err_reduce_body(a, G); }
+__global__ __launch_bounds__(1024) void k_err_reduce(DevGrid a, int G, int from_carry) {
+    err_reduce_body(a, G, from_carry);
+}
 __global__ __launch_bounds__(1024) void k_err_reduce_b(const DevGrid* grids, int G) {
-    err_reduce_body(grids[blockIdx.x], G);
+    err_reduce_body(grids[blockIdx.x], G, 1);
 }
 
 // --------------------------------------------------------------- selection --
@@ -453,8 +459,9 @@ __device__ __forceinline__ void scan_counts_body(const DevGrid& a, int G) {
             cn->n_leaves = L + tl;
             cn->new_samples = ts;
             cn->new_nodes = tl;
+            // the next stale-list error pass starts from it; g_error itself
+            // stays until the next error pass (kdgrid.cu:684-716)
             cn->carry_bits = tm;
-            cn->gmax_bits = tm;  // the next pass's max starts at the carried one
             const int p = cn->passes;
             a.pstats[p & (kMaxPassStats - 1)] = PassStat{L, N, ts, tl, tc, cn->stale, cn->stale_vals, 0};
             cn->passes = p + 1;
@@ -863,13 +870,13 @@ hipError_t launch_arm(Grid& g, hipStream_t st) {
 
 hipError_t launch_errors(Grid& g, hipStream_t st) {
     MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_errors<DD>, dim3(g.err_blocks), dim3(256), 0, st, g.dev));
-    hipLaunchKernelGGL(k_err_reduce, dim3(1), dim3(1024), 0, st, g.dev, g.err_blocks);
+    hipLaunchKernelGGL(k_err_reduce, dim3(1), dim3(1024), 0, st, g.dev, g.err_blocks, 0);
     return hipGetLastError();
 }
 
 hipError_t launch_errors_stale(Grid& g, hipStream_t st) {
     MDAS_DISPATCH_D(g.D, hipLaunchKernelGGL(k_errors_stale<DD>, dim3(g.err_blocks), dim3(256), 0, st, g.dev));
-    hipLaunchKernelGGL(k_err_reduce, dim3(1), dim3(1024), 0, st, g.dev, g.err_blocks);
+    hipLaunchKernelGGL(k_err_reduce, dim3(1), dim3(1024), 0, st, g.dev, g.err_blocks, 1);
     return hipGetLastError();
 }
 

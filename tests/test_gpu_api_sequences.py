@@ -1,8 +1,9 @@
 """Call sequences and misuse through the C ABI: calls before InitialSampling,
 mismatched scenes, out-of-range images and ranges, invalid tiles and launch
 shapes are errors that leave the grid usable; a device-side run followed by
-standalone SamplingPass calls, and repeated runs, equal the oracle doing the
-same (kdgrid.cu:633-779 called in any order)."""
+standalone SamplingPass calls, adaptive passes without a ComputeErrors in
+between, and repeated runs, equal the oracle doing the same
+(kdgrid.cu:633-779 called in any order)."""
 import numpy as np
 import pytest
 
@@ -111,6 +112,42 @@ def test_run_then_standalone_passes(mdas):
     assert g.Validate()
     g.close()
     o.close()
+    sc.close()
+
+
+def test_adaptive_sampling_without_compute_errors(mdas):
+    """AdaptiveSampling keeps using g_error from the last ComputeErrors
+    (kdgrid.cu:684-685 resets it only there) and sees the last pass's
+    children still stale: two adaptive passes in a row, after a standalone
+    ComputeErrors and after a device-side run, equal the oracle doing the
+    same, and so does the maximum error the grid reports."""
+    kind, dim, b, e, W, H, spp, alpha, cand, seed = CASE
+    args, target = _args()
+    sc = mdas.Scene(kind, seed, dim, float(W), float(H))
+    params = sc.params_bytes()
+    for after_run in (False, True):
+        g = mdas.KDGrid(*args)
+        o = pyoracle.OracleGrid(*args)
+        if after_run:
+            assert g.Run(sc, target // 2) == o.run(params, target // 2)
+        else:
+            g.InitialSampling()
+            g.EvalNewSamples(sc)
+            o.initial_sampling()
+            o.eval_scene(params, 0, o.n_samples)
+            g.ComputeErrors()
+            o.compute_errors()
+        assert g.GetMaxError() == o.max_error, after_run
+        for p in range(2):
+            g.AdaptiveSampling()
+            g.EvalNewSamples(sc)
+            assert o.adaptive_sampling() == 0
+            o.eval_scene(params, o.n_samples - o.new_samples, o.new_samples)
+            assert g.GetNewSamples() == o.new_samples, (after_run, p)
+            assert g.GetMaxError() == o.max_error, (after_run, p)
+            _same(g.download_state(), o.state(), f"adaptive pass {p}, after run {after_run}")
+        g.close()
+        o.close()
     sc.close()
 
 
