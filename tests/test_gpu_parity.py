@@ -197,8 +197,14 @@ def test_convert_to_bytes_bit_exact(mdas):
     from pmdas_amd import hip
     rng = np.random.default_rng(1)
     n = 1 << 16
+    t = np.float32(0.0031308)  # the sRGB toe's threshold and its neighbours
+    special = np.array([np.nan, np.inf, -np.inf, -0.0, 0.0, 1e-45, 1.17e-38, 1.0, 1.0000001,
+                        np.nextafter(t, np.float32(0)), t, np.nextafter(t, np.float32(1)),
+                        0.9999999, 255.5 / 256], np.float32)
+    special = np.resize(special, (len(special) + 3) // 4 * 4).reshape(-1, 4)
     px = np.concatenate([rng.random((n, 4), dtype=np.float32) * 1.2 - 0.1,
-                         np.linspace(0, 1, 4096 * 4, dtype=np.float32).reshape(-1, 4)])
+                         np.linspace(0, 1, 4096 * 4, dtype=np.float32).reshape(-1, 4),
+                         special, special[:, ::-1].copy()])
     src = hip.DeviceBuffer(px.nbytes)
     dst = hip.DeviceBuffer(len(px) * 4)
     src.upload(px)
