@@ -110,10 +110,13 @@ int mdas_sampling_density(mdas_handle h, float* host_pixels, int width, int heig
 /* KDGrid::Validate (kdgrid.h:63; kdgrid.cu:848-961), always enabled; *valid = 0/1 */
 int mdas_validate(mdas_handle h, int* valid);
 
-/* Getters (kdgrid.h:65-77).  Pointers are device pointers. */
+/* Getters (kdgrid.h:65-77).  Pointers are device pointers.  After
+ * InitialSampling, mdas_get_new_samples is the initial sample count (the
+ * slice the first integrand launch covers); the reference leaves the
+ * previous pass's count there, and its apps read GetNumberOfSamples at that
+ * point (optixMotionBlur.cpp:352-358). */
 float* mdas_get_sample_coordinates(mdas_handle h);
 float* mdas_get_sample_values(mdas_handle h);
-int mdas_get_dim(mdas_handle h);
 int mdas_get_candidates_num(mdas_handle h);
 int mdas_get_max_node_size(mdas_handle h);
 int mdas_get_new_samples(mdas_handle h);
@@ -123,6 +126,9 @@ int mdas_get_number_of_nodes(mdas_handle h);
 int mdas_get_max_samples(mdas_handle h);
 float mdas_get_scale_x(mdas_handle h);
 float mdas_get_scale_y(mdas_handle h);
+/* Additions: the point dimension, and g_error as the last error pass left it
+ * (kdgrid.cu:21, 286; adaptive passes do not change it). */
+int mdas_get_dim(mdas_handle h);
 float mdas_get_max_error(mdas_handle h);
 
 /* ---- additions without a reference counterpart ---------------------------
