@@ -623,21 +623,33 @@ def main():
                    f"{dev_rec.get('Max Clock Freq. (MHz)', '?')} MHz peak engine clock"),
         "device_record": dev_rec,
     }
-    if not a.no_secondary:
+    def section(key, fn):
+        # a failing secondary line is recorded as such; the headline line
+        # is printed regardless
+        try:
+            line[key] = fn()
+        except Exception as e:  # noqa: BLE001 -- reported in the line itself
+            log(f"[bench] {key} failed: {type(e).__name__}: {e}")
+            line[key] = {"error": f"{type(e).__name__}: {e}"}
+
+    def secondary():
         other = [k for k in WORKLOADS if k != a.workload][0]
         r2 = run_gpu(other, max(1, min(3, a.steps)), 1, log)
-        line["secondary"] = {"workload": other, "value": round(r2["msamples_per_s"], 2),
-                             "unit": "Msamples/s", "mdas_ms_per_step": round(r2["mdas_ms"], 3),
-                             "e2e_msamples_per_s": round(r2["e2e_msamples_per_s"], 2),
-                             "samples": r2["samples"], "passes": r2["passes"]}
+        return {"workload": other, "value": round(r2["msamples_per_s"], 2),
+                "unit": "Msamples/s", "mdas_ms_per_step": round(r2["mdas_ms"], 3),
+                "e2e_msamples_per_s": round(r2["e2e_msamples_per_s"], 2),
+                "samples": r2["samples"], "passes": r2["passes"]}
+
+    if not a.no_secondary:
+        section("secondary", secondary)
     if not a.no_batch:
-        line["batch"] = run_batch(max(1, min(2, a.steps)), log)
+        section("batch", lambda: run_batch(max(1, min(2, a.steps)), log))
     if not a.no_learned:
-        line["learned"] = learned_comparison(log)
+        section("learned", lambda: learned_comparison(log))
     if not a.no_frames:
-        line["frames"] = run_frames(log)
+        section("frames", lambda: run_frames(log))
     if not a.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_shrink, log)
+        section("cpu_baseline", lambda: cpu_baseline(a.workload, a.cpu_shrink, log))
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
