@@ -50,3 +50,24 @@ def test_phase_model_bytes_counts_each_term():
     assert r["achieved"] == 2000.0 and r["frac"] == 0.25
     assert r["whole_run"]["bytes"] == round(sum(b.values()))
     assert abs(r["whole_run"]["achieved"] - sum(b.values()) / 4e-3 / 1e9) < 0.1
+
+
+def test_roofline_traffic_rate(tmp_path):
+    """With a traffic file for the dominant kernel, the line carries the
+    counted traffic over the same launch time and over the copy probe; a
+    file for another kernel is ignored."""
+    import json
+    sys.path.insert(0, ROOT)
+    import bench
+    res = dict(kernels={"k_place": dict(launches=4, ms=2.0, bytes=4e9),
+                        "k_eval": dict(launches=4, ms=9.0, bytes=1e9)}, mdas_ms=4.0)
+    f = tmp_path / "traffic.json"
+    f.write_text(json.dumps({"kernel": "k_place", "hbm_bytes_per_launch": 1.5e9}))
+    r = bench.roofline(res, str(f), copy_gbs=5000.0)
+    assert r["kernel"] == "k_place"  # the integrand is not an MDAS kernel
+    assert r["avg_launch_ms"] == 0.5 and r["traffic"] == 1.5e9
+    assert r["traffic_GBps"] == 3000.0 and r["traffic_frac_of_copy"] == 0.6
+    assert r["frac_of_copy"] == 0.4
+    f.write_text(json.dumps({"kernel": "k_errors", "hbm_bytes_per_launch": 1.5e9}))
+    r = bench.roofline(res, str(f), copy_gbs=5000.0)
+    assert r["traffic"] is None and "traffic_GBps" not in r
