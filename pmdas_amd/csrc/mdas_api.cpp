@@ -101,6 +101,26 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
+// Events bracketing one API call for its log lines, created only when the
+// grid logs or times, and destroyed on every return path.
+struct CallEvents {
+    hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+    CallEvents() = default;
+    CallEvents(const CallEvents&) = delete;
+    CallEvents& operator=(const CallEvents&) = delete;
+    ~CallEvents() {
+        for (hipEvent_t x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+    hipError_t create(int n) {
+        for (int i = 0; i < n; ++i) {
+            const hipError_t r = hipEventCreate(&e[i]);
+            if (r != hipSuccess) return r;
+        }
+        return hipSuccess;
+    }
+};
+
 // Fold the recorded kernel events into per-kernel totals and phase times.
 // Folds the timed launches into per-kernel totals; seq (optional) receives
 // every launch's (name, ms) in launch order.
@@ -261,12 +281,11 @@ int mdas_initial_sampling(mdas_handle h, void* stream) {
     Grid& g = h->g;
     hipStream_t st = S(stream);
     const int L0 = 1 << (g.b * g.D + 2 * g.e);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    CallEvents ev;
     const bool logging = g.logging();
     if (logging) {
-        HIP_TRY(hipEventCreate(&e0));
-        HIP_TRY(hipEventCreate(&e1));
-        HIP_TRY(hipEventRecord(e0, st));
+        HIP_TRY(ev.create(2));
+        HIP_TRY(hipEventRecord(ev.e[0], st));
     }
     // The reference zeroes every seed here (kdgrid.cu:637) so that a leaf
     // created later starts from tea4(index, 0); a split writes seed 0 into
@@ -280,12 +299,10 @@ int mdas_initial_sampling(mdas_handle h, void* stream) {
     g.host.new_nodes = 0;
     g.host.passes = 0;
     if (logging) {
-        HIP_TRY(hipEventRecord(e1, st));
-        HIP_TRY(hipEventSynchronize(e1));
+        HIP_TRY(hipEventRecord(ev.e[1], st));
+        HIP_TRY(hipEventSynchronize(ev.e[1]));
         log_count(g, "INITIAL SAMPLES", 4LL * L0);
-        log_time(g, "INITIAL SAMPLING TIME", elapsed(e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        log_time(g, "INITIAL SAMPLING TIME", elapsed(ev.e[0], ev.e[1]));
     }
     return MDAS_OK;
 }
@@ -295,21 +312,18 @@ int mdas_compute_errors(mdas_handle h, void* stream) {
     Grid& g = h->g;
     if (!g.initialized) return fail(MDAS_ERR_STATE, "InitialSampling has not been called");
     hipStream_t st = S(stream);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    CallEvents ev;
     const bool logging = g.logging();
     if (logging) {
-        HIP_TRY(hipEventCreate(&e0));
-        HIP_TRY(hipEventCreate(&e1));
-        HIP_TRY(hipEventRecord(e0, st));
+        HIP_TRY(ev.create(2));
+        HIP_TRY(hipEventRecord(ev.e[0], st));
     }
     HIP_TRY(mdas::launch_reset_errors(g, 1, st));
     HIP_TRY(mdas::launch_errors(g, st));
     if (logging) {
-        HIP_TRY(hipEventRecord(e1, st));
-        HIP_TRY(hipEventSynchronize(e1));
-        log_time(g, "COMPUTE ERRORS TIME", elapsed(e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        HIP_TRY(hipEventRecord(ev.e[1], st));
+        HIP_TRY(hipEventSynchronize(ev.e[1]));
+        log_time(g, "COMPUTE ERRORS TIME", elapsed(ev.e[0], ev.e[1]));
     }
     return MDAS_OK;
 }
@@ -319,12 +333,11 @@ int mdas_adaptive_sampling(mdas_handle h, void* stream) {
     Grid& g = h->g;
     if (!g.initialized) return fail(MDAS_ERR_STATE, "InitialSampling has not been called");
     hipStream_t st = S(stream);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    CallEvents ev;
     const bool logging = g.logging();
     if (logging) {
-        HIP_TRY(hipEventCreate(&e0));
-        HIP_TRY(hipEventCreate(&e1));
-        HIP_TRY(hipEventRecord(e0, st));
+        HIP_TRY(ev.create(2));
+        HIP_TRY(hipEventRecord(ev.e[0], st));
     }
     HIP_TRY(mdas::launch_arm(g, st));
     HIP_TRY(mdas::launch_adaptive(g, st));
@@ -344,12 +357,10 @@ int mdas_adaptive_sampling(mdas_handle h, void* stream) {
         return fail(s, "device error");
     }
     if (logging) {
-        HIP_TRY(hipEventRecord(e1, st));
-        HIP_TRY(hipEventSynchronize(e1));
+        HIP_TRY(hipEventRecord(ev.e[1], st));
+        HIP_TRY(hipEventSynchronize(ev.e[1]));
         log_count(g, "SAMPLES", g.host.new_samples);
-        log_time(g, "ADAPTIVE SAMPLING TIME", elapsed(e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        log_time(g, "ADAPTIVE SAMPLING TIME", elapsed(ev.e[0], ev.e[1]));
     }
     return MDAS_OK;
 }
@@ -375,20 +386,19 @@ int mdas_integrate(mdas_handle h, float* pixels, int w, int hgt, void* stream) {
     if (rc) return rc;
     hipStream_t st = S(stream);
     const bool logging = g.logging();
-    const bool ev = g.timing || logging;
-    hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
-    if (ev) {
-        for (auto& x : e) HIP_TRY(hipEventCreate(&x));
-        HIP_TRY(hipEventRecord(e[0], st));
+    const bool timed = g.timing || logging;
+    CallEvents ev;
+    if (timed) {
+        HIP_TRY(ev.create(4));
+        HIP_TRY(hipEventRecord(ev.e[0], st));
     }
     HIP_TRY(mdas::recon_prepare(g, w, hgt, st));
-    HIP_TRY(mdas::launch_integrate_gather(g, pixels, w, hgt, st, e[1], e[2]));
-    if (ev) {
-        HIP_TRY(hipEventRecord(e[3], st));
-        HIP_TRY(hipEventSynchronize(e[3]));
-        g.t_recon = elapsed(e[0], e[3]);
-        g.t_sort = elapsed(e[1], e[2]);
-        for (auto& x : e) (void)hipEventDestroy(x);
+    HIP_TRY(mdas::launch_integrate_gather(g, pixels, w, hgt, st, ev.e[1], ev.e[2]));
+    if (timed) {
+        HIP_TRY(hipEventRecord(ev.e[3], st));
+        HIP_TRY(hipEventSynchronize(ev.e[3]));
+        g.t_recon = elapsed(ev.e[0], ev.e[3]);
+        g.t_sort = elapsed(ev.e[1], ev.e[2]);
         int rc2 = sync_counters(g, st);  // the key count, for the byte model
         if (rc2) return rc2;
         if (g.host.recon_overflow)
@@ -934,37 +944,41 @@ int mdas_autotune_recon(mdas_handle h, float* pixels, int w, int hgt, void* stre
     if (rc) return rc;
     static const int cand[][2] = {{8, 8}, {16, 4}, {4, 4}, {8, 2}, {16, 1}};
     hipStream_t st = S(stream);
-    const bool timing = g.timing;
+    // timing off while tuning; on any return the grid keeps its timing flag
+    // and the best tile found so far (its own tile if none was timed)
+    struct Restore {
+        Grid& g;
+        bool timing;
+        int tw, th;
+        ~Restore() {
+            g.timing = timing;
+            g.tile_w = tw;
+            g.tile_h = th;
+        }
+    } restore{g, g.timing, g.tile_w, g.tile_h};
     g.timing = false;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    CallEvents ev;
+    HIP_TRY(ev.create(2));
     float best = 1e30f;
-    int btw = g.tile_w, bth = g.tile_h;
     for (const auto& c : cand) {
         g.tile_w = c[0];
         g.tile_h = c[1];
         HIP_TRY(mdas::recon_prepare(g, w, hgt, st));
         HIP_TRY(mdas::launch_integrate_gather(g, pixels, w, hgt, st, nullptr, nullptr));
-        HIP_TRY(hipEventRecord(e0, st));
+        HIP_TRY(hipEventRecord(ev.e[0], st));
         HIP_TRY(mdas::recon_prepare(g, w, hgt, st));
         HIP_TRY(mdas::launch_integrate_gather(g, pixels, w, hgt, st, nullptr, nullptr));
-        HIP_TRY(hipEventRecord(e1, st));
-        HIP_TRY(hipEventSynchronize(e1));
-        const float ms = elapsed(e0, e1);
+        HIP_TRY(hipEventRecord(ev.e[1], st));
+        HIP_TRY(hipEventSynchronize(ev.e[1]));
+        const float ms = elapsed(ev.e[0], ev.e[1]);
         if (ms < best) {
             best = ms;
-            btw = c[0];
-            bth = c[1];
+            restore.tw = c[0];
+            restore.th = c[1];
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    g.tile_w = btw;
-    g.tile_h = bth;
-    g.timing = timing;
-    if (best_tw) *best_tw = btw;
-    if (best_th) *best_th = bth;
+    if (best_tw) *best_tw = restore.tw;
+    if (best_th) *best_th = restore.th;
     return MDAS_OK;
 }
 
