@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void k_recon_prep(DevGrid a, ReconGeo G, float
     for (int i = beg + threadIdx.x; i < end; i += 256) {
         int4 id4;
         float mn[D], mx[D];
-        load_leaf<D>(a, i, id4, mn, mx);
+        load_leaf<D>(a, i, leaf_count(a, i), id4, mn, mx);
         const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
         // all four loads in flight at once; an empty slot reads sample 0
         float vx[4], vy[4], vz[4];
@@ -567,7 +567,7 @@ __global__ __launch_bounds__(256) void k_scatter(DevGrid a, ReconGeo G, float4* 
     for (int i = blockIdx.x * 256 + threadIdx.x; i < L; i += gridDim.x * 256) {
         int4 id4;
         float mn[D], mx[D];
-        load_leaf<D>(a, i, id4, mn, mx);
+        load_leaf<D>(a, i, leaf_count(a, i), id4, mn, mx);
         const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         int n = 0;

@@ -48,6 +48,8 @@ __device__ __forceinline__ void initial_body(const DevGrid& a, int L0, int blk, 
             // the leaf's line (slots, box, first points) and its overflow row
             store_line<D>(a, i, make_int4(4 * i, 4 * i + 1, 4 * i + 2, 4 * i + 3), off, mx, pts, 4);
             if constexpr (line_pts(D) < 4) {
+                // the overflow row: the points past the line and the slots
+                // kept beside them
                 constexpr int P = line_pts(D), XS = xpts_stride(D);
                 float x[XS];
 #pragma unroll
@@ -55,7 +57,10 @@ __device__ __forceinline__ void initial_body(const DevGrid& a, int L0, int blk, 
 #pragma unroll
                 for (int q = P; q < 4; ++q)
 #pragma unroll
-                    for (int d = 0; d < D; ++d) x[(q - P) * D + d] = pts[q][d];
+                    for (int d = 0; d < D; ++d) x[xpt_at(D, q) + d] = pts[q][d];
+#pragma unroll
+                for (int q = 2; q < 4; ++q)
+                    if (slot_at(D, q) < 0) x[-1 - slot_at(D, q)] = __int_as_float(4 * i + q);
                 float4* xr = reinterpret_cast<float4*>(a.xpts + (size_t)i * XS);
 #pragma unroll
                 for (int k = 0; k < XS / 4; ++k) xr[k] = make_float4(x[4 * k], x[4 * k + 1], x[4 * k + 2], x[4 * k + 3]);
@@ -490,8 +495,33 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
     // the leaf's line: slots, box and its first points, one 128-byte read
     float f[kLineFloats];
     load_line(a, i, f);
-    const int ids[4] = {__float_as_int(f[0]), __float_as_int(f[1]), __float_as_int(f[2]),
-                        __float_as_int(f[3])};
+    // the overflow row (points past the line, the slots beside them) only
+    // when the leaf holds those points
+    constexpr int XS = xpts_stride(D) > 0 ? xpts_stride(D) : 4;
+    float x[XS];
+#pragma unroll
+    for (int t = 0; t < XS; ++t) x[t] = 0.0f;
+    if constexpr (P < 4) {
+        if (cnt > (unsigned)P) {
+            const float4* xr = reinterpret_cast<const float4*>(a.xpts + (size_t)i * xpts_stride(D));
+#pragma unroll
+            for (int t = 0; t < XS / 4; ++t) {
+                const float4 v = xr[t];
+                x[4 * t] = v.x;
+                x[4 * t + 1] = v.y;
+                x[4 * t + 2] = v.z;
+                x[4 * t + 3] = v.w;
+            }
+        }
+    }
+    // slots past the leaf's count are stale: -1
+    int ids[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int at = slot_at(D, q);
+        const int v = at >= 0 ? __float_as_int(f[at]) : __float_as_int(x[-1 - at]);
+        ids[q] = q < (int)cnt ? v : -1;
+    }
     float mn[D], mx[D], diag[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -503,30 +533,10 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
     // in arithmetic); those past the line only when the leaf holds them
     float p[4][D];
 #pragma unroll
-    for (int q = 0; q < P; ++q)
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int d = 0; d < D; ++d) p[q][d] = ids[q] >= 0 ? f[line_pt(D, q) + d] : 0.0f;
-    if constexpr (P < 4) {
-        constexpr int XQ = ((4 - P) * D + 3) / 4;  // float4s of the overflow row in use
-        float x[4 * XQ];
-#pragma unroll
-        for (int t = 0; t < 4 * XQ; ++t) x[t] = 0.0f;
-        if (cnt > (unsigned)P) {
-            const float4* xr = reinterpret_cast<const float4*>(a.xpts + (size_t)i * xpts_stride(D));
-#pragma unroll
-            for (int t = 0; t < XQ; ++t) {
-                const float4 v = xr[t];
-                x[4 * t] = v.x;
-                x[4 * t + 1] = v.y;
-                x[4 * t + 2] = v.z;
-                x[4 * t + 3] = v.w;
-            }
-        }
-#pragma unroll
-        for (int q = P; q < 4; ++q)
-#pragma unroll
-            for (int d = 0; d < D; ++d) p[q][d] = ids[q] >= 0 ? x[(q - P) * D + d] : 0.0f;
-    }
+        for (int d = 0; d < D; ++d)
+            p[q][d] = ids[q] < 0 ? 0.0f : (q < P ? f[line_pt(D, q) + d] : x[xpt_at(D, q) + d]);
     float maxDist = -1.0f;
     float best[D];
 #pragma unroll
@@ -566,12 +576,7 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
 
     if (cnt < 4u) {
         // kdgrid.cu:393-396: insert into the first free slot; the error is kept
-        reinterpret_cast<int*>(a.leaf)[(size_t)i * kLineFloats + cnt] = k;
-        float* dst = cnt < (unsigned)P
-                         ? reinterpret_cast<float*>(a.leaf) + (size_t)i * kLineFloats + line_pt(D, 0) + cnt * D
-                         : a.xpts + (size_t)i * xpts_stride(D) + (cnt - P) * D;
-#pragma unroll
-        for (int d = 0; d < D; ++d) dst[d] = best[d];
+        store_point_slot<D>(a, i, (int)cnt, best, k);
     } else {
         // kdgrid.cu:399-471: split.  LargestAxis scans from the last axis down
         // with a strict '<' (aabb.h:34-45).
@@ -645,11 +650,7 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
         // holds two (D = 6)
         store_line_head2<D>(a, i, make_int4(v[0], v[1], E, E), mn, cmx, sp[0], sp[1]);
         store_line<D>(a, m, make_int4(v[2], v[3], v[4], E), cmn, mx, &sp[2], 3);
-        if constexpr (P < 3) {
-            float* xr = a.xpts + (size_t)m * xpts_stride(D);
-#pragma unroll
-            for (int d = 0; d < D; ++d) xr[d] = sp[4][d];
-        }
+        if constexpr (P < 3) store_point_slot<D>(a, m, 2, sp[4], v[4]);  // D = 6: overflow row
         // what the next error pass needs of both children, coalesced by
         // split rank: their sample indices and box volumes
         a.sdat[2 * (m - L)] = make_int4(v[0], v[1], v[2], v[3]);
@@ -805,7 +806,7 @@ __global__ __launch_bounds__(256) void k_mark_last_children_b(const DevGrid* gri
 template <int D>
 __global__ __launch_bounds__(256) void k_build_pts(DevGrid a, int L) {
     for (int i = blockIdx.x * 256 + threadIdx.x; i < L; i += gridDim.x * 256) {
-        const int4 id = load_slots(a, i);
+        const int4 id = load_slots<D>(a, i, leaf_count(a, i));
         const int ids[4] = {id.x, id.y, id.z, id.w};
 #pragma unroll
         for (int q = 0; q < 4; ++q) {

@@ -247,6 +247,9 @@ int mdas_create(const mdas_config* cfg, mdas_handle* out) {
     }
     e = hipMemset(d.cnt, 0, sizeof(mdas::DevCounters));
     if (e == hipSuccess) e = hipMemset(d.values, 0, sizeof(float) * cs * 3);
+    // overflow rows start as -1 slots (a stale slot is then -1 or an index
+    // written before, never garbage)
+    if (e == hipSuccess && d.xpts) e = hipMemset(d.xpts, 0xff, sizeof(float) * cl * mdas::xpts_stride(D));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         free_grid(g);
@@ -467,13 +470,24 @@ int mdas_download_state(mdas_handle h, float* coords, float* values, int* idx, f
     if (values) HIP_TRY(hipMemcpy(values, g.dev.values, sizeof(float) * N * 3, hipMemcpyDeviceToHost));
     if (err) HIP_TRY(hipMemcpy(err, g.dev.err, sizeof(float) * L, hipMemcpyDeviceToHost));
     if (seed) HIP_TRY(hipMemcpy(seed, g.dev.seed, sizeof(uint32_t) * L, hipMemcpyDeviceToHost));
-    if (idx || box) {  // slots and box from each leaf's line
-        const size_t LF = mdas::kLineFloats;
-        std::vector<float> rec(L * LF);
+    if (idx || box) {  // slots and box from each leaf's line and overflow row
+        const size_t LF = mdas::kLineFloats, XS = (size_t)mdas::xpts_stride(D);
+        std::vector<float> rec(L * LF), xr(L * XS);
+        std::vector<unsigned char> fl(L);
         HIP_TRY(hipMemcpy(rec.data(), g.dev.leaf, sizeof(float) * rec.size(), hipMemcpyDeviceToHost));
+        if (XS) HIP_TRY(hipMemcpy(xr.data(), g.dev.xpts, sizeof(float) * xr.size(), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(fl.data(), g.dev.flags, L, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < L; ++i) {
             const float* r = &rec[i * LF];
-            if (idx) memcpy(idx + 4 * i, r, 16);
+            if (idx) {  // slots past the count read as the reference's empty marker ~D (kdgrid.h:27-30)
+                const int cnt = fl[i] & 7;
+                for (int q = 0; q < 4; ++q) {
+                    const int at = mdas::slot_at(D, q);
+                    int v = ~D;
+                    if (q < cnt) memcpy(&v, at >= 0 ? r + at : &xr[i * XS + (size_t)(-1 - at)], 4);
+                    idx[4 * i + q] = v;
+                }
+            }
             if (box) memcpy(box + i * 2 * D, r + 4, sizeof(float) * 2 * D);
         }
     }
@@ -512,19 +526,26 @@ int mdas_upload_state(mdas_handle h, int n, int l, const float* coords, const fl
         memcpy(s.data(), seed, sizeof(uint32_t) * (size_t)l);
         HIP_TRY(hipMemcpy(g.dev.seed, s.data(), sizeof(uint32_t) * s.size(), hipMemcpyHostToDevice));
     }
-    if (idx) {  // lines with slots and box; the points follow from the coordinates
-        const size_t LF = mdas::kLineFloats;
+    if (idx) {  // lines and overflow rows with slots and box; the points follow from the coordinates
+        const size_t LF = mdas::kLineFloats, XS = (size_t)mdas::xpts_stride(D);
         std::vector<float> rec((size_t)l * LF, 0.0f);
+        std::vector<int> xr((size_t)l * XS, -1);
         std::vector<unsigned char> f((size_t)l);
         for (int i = 0; i < l; ++i) {
             int c = 0;
-            for (int q = 0; q < 4; ++q) c += idx[4 * (size_t)i + q] >= 0 ? 1 : 0;
             float* r = &rec[(size_t)i * LF];
-            memcpy(r, idx + 4 * (size_t)i, 16);
+            for (int q = 0; q < 4; ++q) {
+                const int k = idx[4 * (size_t)i + q];
+                c += k >= 0 ? 1 : 0;
+                const int at = mdas::slot_at(D, q);
+                if (at >= 0) memcpy(r + at, &k, 4);
+                else xr[(size_t)i * XS + (size_t)(-1 - at)] = k;
+            }
             memcpy(r + 4, box + (size_t)i * 2 * D, sizeof(float) * 2 * D);
             f[i] = (unsigned char)c;
         }
         HIP_TRY(hipMemcpy(g.dev.leaf, rec.data(), sizeof(float) * rec.size(), hipMemcpyHostToDevice));
+        if (XS) HIP_TRY(hipMemcpy(g.dev.xpts, xr.data(), sizeof(int) * xr.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(g.dev.flags, f.data(), f.size(), hipMemcpyHostToDevice));
         g.ext_x = g.ext_y = 0.0;
         for (int i = 0; i < l; ++i) {

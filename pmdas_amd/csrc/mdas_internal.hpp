@@ -54,17 +54,31 @@ static const int kMaxPassStats = 4096;  // device ring of per-pass rows (a power
                                         // host drains it before it wraps
 
 // Leaf lines.  Each leaf owns one aligned 128-byte line (kLineFloats floats):
-// the int4 of sample slots, the box (mn[0..D-1], mx[0..D-1]) and the points of
-// its first line_pts(D) slots, D floats each, so that the placement reads and
+// its sample slots, the box (mn[0..D-1], mx[0..D-1]) and the points of its
+// first line_pts(D) slots, D floats each, so that the placement reads and
 // writes a selected leaf in one line.  The points of the remaining slots (D = 5:
 // slot 3; D = 6: slots 2 and 3) sit in a per-leaf overflow row of
 // xpts_stride(D) floats (32 / 64 bytes), touched only by the last insert and
-// the split.  Float offsets in a line: slots 0..3, mn at 4, mx at 4 + D,
-// point q at 4 + 2 D + q D.
+// the split.  Float offsets in a line: slots at slot_at(D, q) (0..3 for
+// D <= 4), mn at 4, mx at 4 + D, point q at 4 + 2 D + q D.
+//
+// A leaf's sample count is the low 3 bits of its flags byte; slots at or past
+// it hold stale indices and are never read.  For D >= 5 the slots that an
+// insert fills sit in the same 32-byte sector as their points, so that an
+// insert writes one sector: D = 5 keeps slot 2 at line float 29 (the line's
+// last sector, with point 2 at 24..28) and slot 3 at overflow float 5 (point 3
+// at 0..4); D = 6 keeps slots 2 and 3 at overflow floats 6 and 14 (points 2
+// and 3 at 0..5 and 8..13).  Slots 0 and 1 are at line floats 0 and 1.
 static const int kLineFloats = 32;
 __host__ __device__ constexpr int line_pts(int D) { return D <= 4 ? 4 : (D == 5 ? 3 : 2); }
 __host__ __device__ constexpr int line_pt(int D, int q) { return 4 + 2 * D + q * D; }
 __host__ __device__ constexpr int xpts_stride(int D) { return D <= 4 ? 0 : (D == 5 ? 8 : 16); }
+// overflow-row float of point q (q >= line_pts(D))
+__host__ __device__ constexpr int xpt_at(int D, int q) { return D == 5 ? 0 : (q == 2 ? 0 : 8); }
+// slot q: a line float (>= 0), or overflow float f encoded as -1 - f
+__host__ __device__ constexpr int slot_at(int D, int q) {
+    return D <= 4 || q < 2 ? q : (D == 5 ? (q == 2 ? 29 : -1 - 5) : (q == 2 ? -1 - 6 : -1 - 14));
+}
 // float4s of a line holding the slots and the box
 __host__ __device__ constexpr int line_box_quads(int D) { return (4 + 2 * D + 3) / 4; }
 

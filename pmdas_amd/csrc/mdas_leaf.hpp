@@ -22,9 +22,23 @@ __device__ __forceinline__ void load_line(const DevGrid& a, int i, float* f) {
     }
 }
 
-// Slots and box only: the line's first line_box_quads(D) float4.
+// The leaf's sample count (the low bits of its flags byte).
+__device__ __forceinline__ int leaf_count(const DevGrid& a, int i) { return (int)(a.flags[i] & 7u); }
+
+// Slot q of leaf i (q < its count) where slot_at(D, q) keeps it.
 template <int D>
-__device__ __forceinline__ void load_leaf(const DevGrid& a, int i, int4& id, float* mn, float* mx) {
+__device__ __forceinline__ int load_slot(const DevGrid& a, int i, int q) {
+    const int at = slot_at(D, q);
+    return at >= 0 ? reinterpret_cast<const int*>(a.leaf)[(size_t)i * kLineFloats + at]
+                   : reinterpret_cast<const int*>(a.xpts)[(size_t)i * xpts_stride(D) + (-1 - at)];
+}
+
+// Slots (the first cnt; -1 after them) and box: the line's first
+// line_box_quads(D) float4, and the slots kept past them (D >= 5) only when
+// the count reaches them.
+template <int D>
+__device__ __forceinline__ void load_leaf(const DevGrid& a, int i, int cnt, int4& id, float* mn,
+                                          float* mx) {
     constexpr int Q = line_box_quads(D);
     const float4* p = a.leaf + (size_t)i * (kLineFloats / 4);
     float b[4 * Q];
@@ -36,7 +50,15 @@ __device__ __forceinline__ void load_leaf(const DevGrid& a, int i, int4& id, flo
         b[4 * k + 2] = v.z;
         b[4 * k + 3] = v.w;
     }
-    id = make_int4(__float_as_int(b[0]), __float_as_int(b[1]), __float_as_int(b[2]), __float_as_int(b[3]));
+    int s[4] = {__float_as_int(b[0]), __float_as_int(b[1]), __float_as_int(b[2]), __float_as_int(b[3])};
+    if constexpr (D >= 5) {
+        s[2] = cnt > 2 ? load_slot<D>(a, i, 2) : -1;
+        s[3] = cnt > 3 ? load_slot<D>(a, i, 3) : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (q >= cnt) s[q] = -1;
+    id = make_int4(s[0], s[1], s[2], s[3]);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         mn[d] = b[4 + d];
@@ -47,32 +69,66 @@ __device__ __forceinline__ void load_leaf(const DevGrid& a, int i, int4& id, flo
 template <int D>
 __device__ __forceinline__ void load_box(const DevGrid& a, int i, float* mn, float* mx) {
     int4 id;
-    load_leaf<D>(a, i, id, mn, mx);
+    load_leaf<D>(a, i, 0, id, mn, mx);
 }
 
-__device__ __forceinline__ int4 load_slots(const DevGrid& a, int i) {
-    const float4 q0 = a.leaf[(size_t)i * (kLineFloats / 4)];
-    return make_int4(__float_as_int(q0.x), __float_as_int(q0.y), __float_as_int(q0.z),
-                     __float_as_int(q0.w));
+// The slots of leaf i (the first cnt; -1 after them).
+template <int D>
+__device__ __forceinline__ int4 load_slots(const DevGrid& a, int i, int cnt) {
+    int s[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s[q] = q < cnt ? load_slot<D>(a, i, q) : -1;
+    return make_int4(s[0], s[1], s[2], s[3]);
 }
 
 // Address of point q (slot order) of leaf i: in its line or its overflow row.
 template <int D>
 __device__ __forceinline__ float* point_addr(const DevGrid& a, int i, int q) {
     return q < line_pts(D) ? reinterpret_cast<float*>(a.leaf) + (size_t)i * kLineFloats + line_pt(D, q)
-                           : a.xpts + (size_t)i * xpts_stride(D) + (q - line_pts(D)) * D;
+                           : a.xpts + (size_t)i * xpts_stride(D) + xpt_at(D, q);
 }
 
-// Writes a whole line: slots, box and the first points (pts[q] for q <
-// min(npts, line_pts(D))), zeros elsewhere, as 8 float4 stores.
+// An insert (kdgrid.cu:392-396): point pt and sample index k into slot c of
+// leaf i.  For D >= 5 and c >= 2 the slot shares the point's 32-byte sector
+// (slot_at), written as two whole float4 with the sector's spare floats
+// zeroed; otherwise the slot and the point are written where they lie.
+template <int D>
+__device__ __forceinline__ void store_point_slot(const DevGrid& a, int i, int c, const float* pt, int k) {
+    if constexpr (D >= 5) {
+        if (c >= 2) {
+            float w[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) w[t] = 0.0f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) w[d] = pt[d];
+            w[D] = __int_as_float(k);
+            // D = 5: line floats 24..31 (c = 2) or overflow floats 0..7 (c = 3);
+            // D = 6: overflow floats 0..7 (c = 2) or 8..15 (c = 3)
+            float* base = c < line_pts(D) ? reinterpret_cast<float*>(a.leaf) + (size_t)i * kLineFloats + line_pt(D, c)
+                                          : a.xpts + (size_t)i * xpts_stride(D) + xpt_at(D, c);
+            float4* q4 = reinterpret_cast<float4*>(base);
+            q4[0] = make_float4(w[0], w[1], w[2], w[3]);
+            q4[1] = make_float4(w[4], w[5], w[6], w[7]);
+            return;
+        }
+    }
+    reinterpret_cast<int*>(a.leaf)[(size_t)i * kLineFloats + c] = k;
+    float* dst = point_addr<D>(a, i, c);
+#pragma unroll
+    for (int d = 0; d < D; ++d) dst[d] = pt[d];
+}
+
+// Writes a whole line: the slots kept in it, box and the first points
+// (pts[q] for q < min(npts, line_pts(D))), zeros elsewhere, as 8 float4
+// stores.  Slots kept in the overflow row (D >= 5) are the caller's.
 template <int D>
 __device__ __forceinline__ void store_line(const DevGrid& a, int i, int4 id, const float* mn,
                                            const float* mx, const float (*pts)[D], int npts) {
     float f[kLineFloats];
     f[0] = __int_as_float(id.x);
     f[1] = __int_as_float(id.y);
-    f[2] = __int_as_float(id.z);
-    f[3] = __int_as_float(id.w);
+    f[2] = __int_as_float(D <= 4 ? id.z : -1);
+    f[3] = __int_as_float(D <= 4 ? id.w : -1);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         f[4 + d] = mn[d];
@@ -85,6 +141,7 @@ __device__ __forceinline__ void store_line(const DevGrid& a, int i, int4 id, con
 #pragma unroll
         for (int d = 0; d < D; ++d)
             if (q < npts) f[line_pt(D, q) + d] = pts[q][d];
+    if constexpr (D == 5) f[slot_at(5, 2)] = __int_as_float(id.z);  // beside point 2
     float4* p = a.leaf + (size_t)i * (kLineFloats / 4);
 #pragma unroll
     for (int k = 0; k < kLineFloats / 4; ++k)
@@ -93,7 +150,7 @@ __device__ __forceinline__ void store_line(const DevGrid& a, int i, int4 id, con
 
 // Writes the line's first 4 + 4 D floats: slots, box and the first two
 // points (a left child keeps two), whole float4s; the rest of the line keeps
-// its old contents, which the slots mark empty.
+// its old contents, past the leaf's count of 2.
 template <int D>
 __device__ __forceinline__ void store_line_head2(const DevGrid& a, int i, int4 id, const float* mn,
                                                  const float* mx, const float* p0, const float* p1) {
@@ -102,8 +159,8 @@ __device__ __forceinline__ void store_line_head2(const DevGrid& a, int i, int4 i
     float f[NF];
     f[0] = __int_as_float(id.x);
     f[1] = __int_as_float(id.y);
-    f[2] = __int_as_float(id.z);
-    f[3] = __int_as_float(id.w);
+    f[2] = __int_as_float(D <= 4 ? id.z : -1);
+    f[3] = __int_as_float(D <= 4 ? id.w : -1);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         f[4 + d] = mn[d];
@@ -117,11 +174,6 @@ __device__ __forceinline__ void store_line_head2(const DevGrid& a, int i, int4 i
         p[k] = make_float4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);
 }
 
-// kdgrid.cu:237-266: relative-L1 error of a leaf's samples times its volume.
-// Also returns the leaf's sample values by slot, its box and volume for
-// callers that derive more statistics from the same reads.  Occupied slots
-// are always 0..n-1: slots fill in order and a split writes both children
-// from slot 0.
 // Stopping rule of the device-side loop (optixMotionBlur.cpp:781): no
 // further pass once N >= min(target, max_samples).  When the limit is the
 // capacity itself (target >= max_samples) also none once a pass could
@@ -215,11 +267,16 @@ __device__ __forceinline__ float box_volume(const float* mn, const float* mx) {
     return vol;
 }
 
+// kdgrid.cu:237-266: relative-L1 error of a leaf's samples times its volume.
+// Also returns the leaf's sample values by slot, its box and volume for
+// callers that derive more statistics from the same reads.  Occupied slots
+// are always 0..n-1: slots fill in order and a split writes both children
+// from slot 0.
 template <int D>
 __device__ __forceinline__ float leaf_error_s(const DevGrid& a, int i, LeafSamples& ls, float* mn,
                                               float* mx) {
     int4 id;
-    load_leaf<D>(a, i, id, mn, mx);
+    load_leaf<D>(a, i, leaf_count(a, i), id, mn, mx);
     const int ids[4] = {id.x, id.y, id.z, id.w};
     return leaf_error_ids(a, ids, box_volume<D>(mn, mx), ls);
 }
