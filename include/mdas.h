@@ -209,7 +209,9 @@ int mdas_autotune_recon(mdas_handle h, float* pixels, int width, int height, voi
 
 /* State transfer for tests and tools (synchronous).
  * idx[L*4], box[L*2*dim] (per leaf: mn[dim] then mx[dim]), err[L], seed[L],
- * coords[N*dim], values[N*3]; NULL pointers are skipped. */
+ * coords[N*dim], values[N*3]; NULL pointers are skipped.  A leaf's occupied
+ * slots come first; an upload takes any negative index as empty, a download
+ * writes the reference's empty marker ~dim (kdgrid.h:27-30) there. */
 int mdas_download_state(mdas_handle h, float* coords, float* values, int* idx, float* box,
                         float* err, uint32_t* seed);
 int mdas_upload_state(mdas_handle h, int n_samples, int n_leaves, const float* coords,
