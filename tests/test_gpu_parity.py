@@ -297,12 +297,16 @@ def test_refused_pass_mid_run_then_retry(mdas):
     sc.close()
 
 
-def test_upload_state_then_pass(mdas):
-    """A pass on an uploaded mid-run state matches the oracle on the same state."""
-    g, o, sc, target = _make(mdas, 3, 5, 1, 2, 48, 32, 8, 0.25, 4)
+@pytest.mark.parametrize("kind,dim", [(1, 3), (2, 4), (3, 5), (4, 6)])
+def test_upload_state_then_pass(mdas, kind, dim):
+    """An uploaded mid-run state reads back as uploaded (slots, wherever the
+    line layout keeps them, with the reference's empty markers), and a pass
+    on it matches the oracle on the same state."""
+    g, o, sc, target = _make(mdas, kind, dim, 1, 1 if dim >= 5 else 2, 48, 32, 8, 0.25, 4)
     o.run(sc.params_bytes(), target // 2)
     st = o.state()
     g.upload_state(st["coords"], st["values"], st["idx"], st["box"], st["err"], st["seed"])
+    _same_state(g.download_state(), st, "uploaded")
     g.ComputeErrors()
     o.compute_errors()
     g.AdaptiveSampling()
