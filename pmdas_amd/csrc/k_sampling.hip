@@ -397,19 +397,47 @@ static int select_cpw(int chunks) { return chunks > 4096 && chunks % 4096 == 0 ?
 __device__ __forceinline__ void scan_counts_body(const DevGrid& a, int G) {
     __shared__ int ls[16], ll[16], lc[16];
     __shared__ unsigned lm[16];
-    if (a.cnt->done) return;
+    DevCounters* cn = a.cnt;
+    if (cn->done) return;
+    // thread 0's counter reads go out together with the partials' loads
+    // (no other kernel runs between the error pass and this one)
+    int N = 0, L = 0, P = 0, st = 0, sv = 0;
+    if (threadIdx.x == 0) {
+        N = cn->n_samples;
+        L = cn->n_leaves;
+        P = cn->passes;
+        st = cn->stale;
+        sv = cn->stale_vals;
+    }
     const int per = (G + 1023) / 1024;
     const int g0 = threadIdx.x * per;
+    // up to R partials per thread (G <= 8192 chunks) stay in registers for
+    // the offsets written below; more are read again there
+    constexpr int R = 8;
+    int4 pr[R];
     int s = 0, l = 0, c = 0;
     unsigned mx = 0;
-    for (int k = 0; k < per; ++k)
-        if (g0 + k < G) {
-            int4 p = a.part[g0 + k];
-            s += p.x;
-            l += p.y;
-            c += p.z;
-            mx = max(mx, (unsigned)p.w);
+    if (per <= R) {
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            pr[k] = k < per && g0 + k < G ? a.part[g0 + k] : make_int4(0, 0, 0, 0);
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            s += pr[k].x;
+            l += pr[k].y;
+            c += pr[k].z;
+            mx = max(mx, (unsigned)pr[k].w);
         }
+    } else {
+        for (int k = 0; k < per; ++k)
+            if (g0 + k < G) {
+                int4 p = a.part[g0 + k];
+                s += p.x;
+                l += p.y;
+                c += p.z;
+                mx = max(mx, (unsigned)p.w);
+            }
+    }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int xs = s, xl = l, xc = c;
     unsigned xm = mx;
@@ -444,16 +472,24 @@ __device__ __forceinline__ void scan_counts_body(const DevGrid& a, int G) {
         tm = max(tm, lm[k]);
     }
     int rs = ps + xs - s, rl = pl + xl - l;
-    for (int k = 0; k < per; ++k)
-        if (g0 + k < G) {
-            int4 p = a.part[g0 + k];
-            a.part_off[g0 + k] = make_int2(rs, rl);
-            rs += p.x;
-            rl += p.y;
-        }
+    if (per <= R) {
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (k < per && g0 + k < G) {
+                a.part_off[g0 + k] = make_int2(rs, rl);
+                rs += pr[k].x;
+                rl += pr[k].y;
+            }
+    } else {
+        for (int k = 0; k < per; ++k)
+            if (g0 + k < G) {
+                int4 p = a.part[g0 + k];
+                a.part_off[g0 + k] = make_int2(rs, rl);
+                rs += p.x;
+                rl += p.y;
+            }
+    }
     if (threadIdx.x == 0) {
-        DevCounters* cn = a.cnt;
-        const int N = cn->n_samples, L = cn->n_leaves;
         if ((long long)N + ts > a.cap_s || (long long)L + tl > a.cap_l) {
             cn->status = MDAS_ERR_CAPACITY_DEV;
             cn->done = 1;
@@ -467,9 +503,8 @@ __device__ __forceinline__ void scan_counts_body(const DevGrid& a, int G) {
             // the next stale-list error pass starts from it; g_error itself
             // stays until the next error pass (kdgrid.cu:684-716)
             cn->carry_bits = tm;
-            const int p = cn->passes;
-            a.pstats[p & (kMaxPassStats - 1)] = PassStat{L, N, ts, tl, tc, cn->stale, cn->stale_vals, 0};
-            cn->passes = p + 1;
+            a.pstats[P & (kMaxPassStats - 1)] = PassStat{L, N, ts, tl, tc, st, sv, 0};
+            cn->passes = P + 1;
         }
         cn->sel_cnt = 0;
         cn->stale = 0;
