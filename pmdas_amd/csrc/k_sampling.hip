@@ -3,11 +3,12 @@
 // median split) with indices allocated by a deterministic device-wide scan.
 //
 // Layout (HBM): sample coordinates AoS float[D] and values float3 (the
-// integrand contract); one 48/64-byte record per leaf (int4 slots + box) so a
-// leaf touched at a random index costs one cache-line segment; errors, LCG
-// states and a one-byte flag word per leaf as separate streams for the
-// per-pass sweep over all leaves.  Cross-workgroup reductions go through
-// per-workgroup partials (no same-address atomics).
+// integrand contract); one 128-byte line per leaf (slots, box and its first
+// points) plus an overflow row for the other points (D >= 5), so a leaf
+// touched at a random index costs one line; errors, LCG states and a one-byte
+// flag word (the sample count) per leaf as separate streams for the per-pass
+// sweep over all leaves.  Cross-workgroup reductions go through per-workgroup
+// partials (no same-address atomics).
 #include "mdas_leaf.hpp"
 
 #pragma clang fp contract(off)

@@ -1,6 +1,8 @@
-// Access to the leaf lines (layout in mdas_internal.hpp): the int4 of sample
-// slots, the box (mn[0..D-1], mx[0..D-1]) and the first line_pts(D) points in
-// one aligned 128-byte line per leaf; the other points in the overflow rows.
+// Access to the leaf lines (layout in mdas_internal.hpp): the sample slots,
+// the box (mn[0..D-1], mx[0..D-1]) and the first line_pts(D) points in one
+// aligned 128-byte line per leaf; the other points, and for D >= 5 the slots
+// beside them, in the overflow rows.  A leaf's count (its flags byte) says
+// which slots hold samples.
 #pragma once
 #include "mdas_device.hpp"
 #include "mdas_internal.hpp"
