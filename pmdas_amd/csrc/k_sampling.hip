@@ -681,7 +681,7 @@ __device__ __forceinline__ void place_leaf(const DevGrid& a, int i, int k, int m
                 sp[j][d] = x;
             }
         // the left child in the parent's line (slots, box, two points: whole
-        // float4s, the rest stays and is marked empty), the right child's
+        // float4s, the rest stays, past the child's count of 2), the right child's
         // whole line, and its third point in its overflow row when the line
         // holds two (D = 6)
         store_line_head2<D>(a, i, make_int4(v[0], v[1], E, E), mn, cmx, sp[0], sp[1]);

@@ -86,7 +86,7 @@ __host__ __device__ constexpr int line_box_quads(int D) { return (4 + 2 * D + 3)
 struct DevGrid {
     float* coords;         // [cap_s * D]  AoS (the integrand contract)
     float* values;         // [cap_s * 3]  float3 AoS
-    float4* leaf;          // [cap_l * 8]  leaf lines: slots (~D = empty), box, first points
+    float4* leaf;          // [cap_l * 8]  leaf lines: slots, box, first points
     float* xpts;           // [cap_l * xpts_stride(D)]  the points of the other slots (D >= 5)
     float* err;            // [cap_l]      < 0 = stale
     unsigned* seed;        // [cap_l]      LCG state, 0 = unset
