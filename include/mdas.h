@@ -230,7 +230,10 @@ int mdas_get_recon_keys(mdas_handle h, uint64_t* out, long long capacity, long l
  * alone.  The grids and scenes must outlive the batch.  A grid's learned
  * mode (mdas_set_learned / _exponent) is taken at mdas_batch_create: the
  * batched error pass then computes features and network scores for the
- * learned members and the plain error for the others. */
+ * learned members and the plain error for the others.  With use_graph and a
+ * NULL stream (which cannot be captured) the batch runs on a stream of its
+ * own that waits for earlier work on the null stream and that later work on
+ * it waits for. */
 typedef struct mdas_batch* mdas_batch_handle;
 int mdas_batch_create(mdas_handle* grids, mdas_scene_handle* scenes, int n, mdas_batch_handle* out);
 int mdas_batch_destroy(mdas_batch_handle b);
